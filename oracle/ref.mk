@@ -1,0 +1,42 @@
+# Builds the reference T-FSM miners, unmodified, from their own sources under
+# REF (default /root/reference), into oracle/_ref/.  The products are test and
+# baseline infrastructure only (golden pattern sets; the timed CPU baseline).
+#
+#   make -f oracle/ref.mk            # from the repository root
+#
+# -march=x86-64-v3 instead of the reference's -march=native: the AVX2 kernels
+# in intersection/computesetintersection.cpp need AVX2, and the binary must
+# also run on other x86-64 hosts than the one it was built on.
+REF     ?= /root/reference
+OUT     ?= oracle/_ref
+CXX     ?= g++
+CXXFLAGS = -std=c++14 -O3 -march=x86-64-v3 -fopenmp -w
+
+all: $(OUT)/run_parallel $(OUT)/run_serial
+
+$(OUT)/parallel_intersection.o: $(REF)/parallel/intersection/computesetintersection.cpp
+	@mkdir -p $(OUT)
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/parallel_run.o: $(REF)/parallel/run.cpp $(wildcard $(REF)/parallel/*.h)
+	@mkdir -p $(OUT)
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/run_parallel: $(OUT)/parallel_run.o $(OUT)/parallel_intersection.o
+	$(CXX) $(CXXFLAGS) $^ -o $@ -lpthread
+
+$(OUT)/serial_intersection.o: $(REF)/serial/intersection/computesetintersection.cpp
+	@mkdir -p $(OUT)
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/serial_run.o: $(REF)/serial/run.cpp $(wildcard $(REF)/serial/*.h)
+	@mkdir -p $(OUT)
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/run_serial: $(OUT)/serial_run.o $(OUT)/serial_intersection.o
+	$(CXX) $(CXXFLAGS) $^ -o $@
+
+clean:
+	rm -f $(OUT)/run_parallel $(OUT)/run_serial $(OUT)/*.o
+
+.PHONY: all clean
