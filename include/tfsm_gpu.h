@@ -1,0 +1,179 @@
+/* C ABI of the MI355X (gfx950) MNI support-evaluation library, libtfsm_gpu.so.
+ *
+ * The reference (lyuheng/T-FSM, parallel build) has no plugin or FFI layer:
+ * the seam it exposes is two C++ calls inside one translation unit, and each
+ * entry point below replaces one of them:
+ *
+ *   per pattern, at activation (parallel/comper.h:1655-1700):
+ *       GMatchEngine::set(&grami.pruned_graph, pattern);
+ *       DPisoFilter(false, freq);                 // gmatch.h:417-472
+ *       buildTable(tc->edge_matrix);              // gmatch.h:995-1130
+ *       generateGQLQueryPlan(u, order[u]);        // gmatch.h:692-727
+ *       generateBN(...);                          // gmatch.h:803-828
+ *   per task (comper.h:431-533, 1181-1300):
+ *       int GMatchEngine::execute(u, v_idx, embedding, ms);   // gmatch.h:205-222
+ *       followed by the MNI bookkeeping of Comper::compute / postprocess.
+ *
+ * Here one call evaluates a whole pattern (tfsm_eval_pattern: filter, table,
+ * plans and every (u, v) existence check with the early-frequent and
+ * early-infrequent exits) or every pattern of one extension level
+ * (tfsm_eval_level).  Return codes replace the reference's magic ints
+ * MATCH_FOUND -1 / MATCH_NOT_FOUND -2 / TIMEOUT -3 (comper.h:30-32) and its
+ * exit(-1) on I/O errors: every call returns a tfsm_status, there is no
+ * timeout status (every (u, v) gets a final verdict), and nothing exits or
+ * throws across the boundary.
+ *
+ * Threading: a context is not thread-safe; one host thread drives it.
+ * Ownership: input arrays are caller-owned and only read during the call;
+ * device memory (graph, scratch, retained domains) is context-owned.
+ */
+#ifndef TFSM_GPU_H
+#define TFSM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TFSM_MAX_PATTERN_VERTICES 16
+#define TFSM_MAX_PATTERN_EDGES 64
+
+typedef enum {
+    TFSM_OK = 0,
+    TFSM_ERR_ARGUMENT = -1,     /* malformed input (pattern, sizes, null pointers) */
+    TFSM_ERR_HIP = -2,          /* a HIP runtime call failed */
+    TFSM_ERR_NO_DEVICE = -3,    /* no usable GPU */
+    TFSM_ERR_LIMIT = -4,        /* input beyond a compiled limit */
+    TFSM_ERR_HANDLE = -5        /* unknown or released domain handle */
+} tfsm_status;
+
+/* Pruned data graph (the output of the reference's construct_freq_graph,
+ * graph.h:685-803): dense vertex ids, per-vertex adjacency ascending by
+ * neighbour id, labels given as dense indices. */
+typedef struct {
+    uint32_t num_vertices;
+    uint64_t num_adj;               /* directed entries = 2 |E| */
+    const uint32_t *offsets;        /* num_vertices + 1 */
+    const uint32_t *nbrs;           /* num_adj */
+    const uint32_t *elabels;        /* num_adj, dense edge label of each entry */
+    const uint32_t *vlabels;        /* num_vertices, dense vertex label */
+    uint32_t num_vlabels;
+    uint32_t num_elabels;
+} tfsm_csr;
+
+/* A connected pattern (the reference's Pattern, graph.h:419-550). */
+typedef struct {
+    uint32_t num_vertices;
+    uint32_t num_edges;
+    uint32_t vlabel[TFSM_MAX_PATTERN_VERTICES];     /* dense labels */
+    uint32_t edge_a[TFSM_MAX_PATTERN_EDGES];
+    uint32_t edge_b[TFSM_MAX_PATTERN_EDGES];
+    uint32_t edge_label[TFSM_MAX_PATTERN_EDGES];    /* dense labels */
+} tfsm_pattern;
+
+enum {
+    TFSM_MODE_CAPPED = 0,   /* reference semantics: stop at the frequent / infrequent verdict */
+    TFSM_MODE_EXACT = 1     /* decide every (u, v): exact MNI support */
+};
+
+enum {
+    TFSM_ORDER_REFERENCE = 0,   /* u ascending, candidates in domain order (comper.h:1181-1239) */
+    TFSM_ORDER_LEARNED = 1      /* ordered by the ranker installed with tfsm_ranker_set */
+};
+
+typedef struct {
+    uint32_t mode;              /* TFSM_MODE_* */
+    uint32_t retain_domains;    /* keep a frequent pattern's child domains on the device */
+    uint32_t task_order;        /* TFSM_ORDER_* */
+    uint32_t trace;             /* record per-task verdict and step count (tfsm_read_trace) */
+    uint32_t block_waves;       /* waves per workgroup of the search kernel (0: tuned default) */
+    uint32_t grid_waves_per_cu; /* resident waves per CU of the search kernel (0: tuned default) */
+    uint32_t split_steps;       /* reserved (0) */
+    uint32_t reserved;
+} tfsm_eval_options;
+
+/* decided_by */
+enum { TFSM_DECIDED_FILTER = 0, TFSM_DECIDED_EXPRESS = 1, TFSM_DECIDED_SEARCH = 2 };
+
+typedef struct {
+    int32_t frequent;
+    int32_t decided_by;
+    uint32_t support_lower;     /* min over u of proven-valid candidates */
+    uint32_t support_upper;     /* min over u of candidates not proven invalid */
+    uint32_t domain_size[TFSM_MAX_PATTERN_VERTICES];    /* |C(u)| after filtering */
+    uint32_t valid_count[TFSM_MAX_PATTERN_VERTICES];
+    uint32_t invalid_count[TFSM_MAX_PATTERN_VERTICES];
+    uint64_t searches;          /* existence searches started */
+    uint64_t verdicts;          /* final (u, v) verdicts, by search or by propagation */
+    uint64_t domain_handle;     /* retained child domains (0: none) */
+} tfsm_pattern_result;
+
+typedef struct tfsm_ctx tfsm_ctx;
+
+/* Upload the graph to device 0 (current HIP device).
+ * Replaces Worker::load_data's hand-off of grami.pruned_graph (worker.h:47-55). */
+int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out);
+int tfsm_ctx_destroy(tfsm_ctx *ctx);
+
+/* One pattern, one launch sequence (comper.h:1655-1700 + 431-533 + 1181-1300).
+ * parent_handle: retained domains of the DFS-code parent (0: label buckets, as
+ * for the children of size-2 patterns, worker.h:57-106); the first
+ * `parent vertices` candidate lists start from it (gmatch.h:335-359). */
+int tfsm_eval_pattern(tfsm_ctx *ctx, const tfsm_pattern *p, uint64_t parent_handle, uint32_t freq,
+                      const tfsm_eval_options *opt, tfsm_pattern_result *out);
+
+/* Every pattern of one extension level in one fused launch sequence. */
+int tfsm_eval_level(tfsm_ctx *ctx, const tfsm_pattern *ps, const uint64_t *parent_handles, uint32_t n,
+                    uint32_t freq, const tfsm_eval_options *opt, tfsm_pattern_result *outs);
+
+/* Child domain of pattern vertex u (C(u) minus proven-invalid candidates,
+ * comper.h:1001-1018) of a retained handle; *size gets the full size. */
+int tfsm_domain_read(tfsm_ctx *ctx, uint64_t handle, uint32_t u, uint32_t *out, uint32_t cap, uint32_t *size);
+int tfsm_domain_release(tfsm_ctx *ctx, uint64_t handle);
+
+/* Inspection of the last tfsm_eval_pattern call (tests and tracing):
+ *   tfsm_last_domain  : C(u) after filtering (sorted ids), returns size
+ *   tfsm_last_states  : per candidate of u: 0 undecided, 1 valid, 2 invalid
+ *   tfsm_last_aux     : auxiliary table a->b (rows over C(a), ascending
+ *                       indices into C(b)), the reference's edge_matrix[a][b];
+ *                       returns entries or -1 when (a, b) is not a pattern edge
+ *   tfsm_last_order   : matching order starting at u */
+int64_t tfsm_last_domain(tfsm_ctx *ctx, uint32_t u, uint32_t *out, uint32_t cap);
+int64_t tfsm_last_states(tfsm_ctx *ctx, uint32_t u, uint8_t *out, uint32_t cap);
+int64_t tfsm_last_aux(tfsm_ctx *ctx, uint32_t a, uint32_t b, uint32_t *off, uint32_t off_cap, uint32_t *idx,
+                      uint32_t idx_cap);
+int tfsm_last_order(tfsm_ctx *ctx, uint32_t u, uint32_t *order);
+
+/* Per-task trace of the last call with opt.trace set, in task order of the
+ * reference (pattern, u, candidate index): verdict (0 skipped, 1 valid,
+ * 2 invalid, 3 aborted) in bits 30-31 and DFS steps in bits 0-29. */
+int64_t tfsm_read_trace(tfsm_ctx *ctx, uint32_t *out, uint64_t cap);
+
+/* Timing and traffic of the last eval call, measured with HIP events on the
+ * library's stream.  Algorithmic bytes count each input byte the phase must
+ * read and each output byte it must write once:
+ *   table  : per directed pattern edge a->b, 8 B of segment bounds and 4 B per
+ *            scanned neighbour id for every v in C(a), plus 4 B per row offset
+ *            and per table entry written;
+ *   search : per (u, v) verdict, sum over pattern neighbours u' of u of
+ *            8 B of row bounds + 4 B per entry of the depth-1 row A[u->u'][v]
+ *            (taken at the mean row length of u). */
+typedef struct {
+    float filter_ms;            /* LDF + DP-iso passes + compaction (incl. host round trips) */
+    float table_ms;             /* count + scan + fill */
+    float search_ms;            /* existence-search kernel launch alone */
+    uint32_t search_launches;
+    uint64_t table_bytes;
+    uint64_t search_bytes;
+} tfsm_call_stats;
+int tfsm_last_stats(tfsm_ctx *ctx, tfsm_call_stats *out);
+
+const char *tfsm_status_string(int status);
+const char *tfsm_last_error(tfsm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

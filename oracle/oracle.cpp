@@ -435,7 +435,9 @@ extern "C" orc_eval *orc_eval_pattern(const orc_graph *gp, int nv, const int32_t
     r->nv = nv;
     // activation, comper.h:1614-1700
     if (!ldf(g, q, parent_nv, parent_off, parent_dom, r->C)) {
-        r->C.resize(nv);
+        // an empty candidate set empties every domain of a connected pattern
+        // under refinement; the reference stops here (gmatch.h:387-391)
+        r->C.assign(nv, {});
         r->state.resize(nv);
         for (int u = 0; u < nv; ++u) r->state[u].assign(r->C[u].size(), 0);
         r->decided_by = 0;

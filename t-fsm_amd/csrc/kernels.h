@@ -1,0 +1,111 @@
+// Device-side data layout and kernels of the MNI evaluation library (gfx950).
+//
+// Layout in HBM:
+//   graph  : label-partitioned CSR.  Adjacency of v is sorted by
+//            (label(w), edge label, w); seg[v*(K+1)+k] is the absolute start
+//            of key k = label*EL + elabel and seg[v*(K+1)+K] the end of v's
+//            list.  NLF(v, l) (graph.h:850-866 of the reference) is
+//            seg[v][(l+1)EL] - seg[v][l EL]; a label/edge-label restricted
+//            neighbour scan is one contiguous range.
+//   domain : per pattern vertex, a membership bitmap over graph vertices
+//            (W = ceil(n/32) words), a rank array (prefix popcounts per word)
+//            and the sorted candidate list C(u).  rank replaces the
+//            reference's |V|-sized flag[] map (gmatch.h:1020-1030).
+//   table  : per directed pattern edge a->b ("slot"), row offsets over C(a)
+//            and ascending indices into C(b) in one shared pool: the
+//            reference's edge_matrix[a][b] (types.h:67-90, gmatch.h:995-1130).
+//   verdict: valid / invalid bitmaps over candidate indices of each u plus
+//            counters, replacing domain_matches / non_candidates
+//            (comper.h:104-298).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tfsm_gpu.h"
+
+namespace tfsm_dev {
+
+constexpr int kMaxV = TFSM_MAX_PATTERN_VERTICES;
+constexpr int kMaxE = TFSM_MAX_PATTERN_EDGES;
+constexpr int kMaxSlots = 2 * kMaxE;
+constexpr int kBlock = 256;            // threads per block of every kernel
+
+struct DevGraph {
+    uint32_t n;        // vertices
+    uint32_t W;        // bitmap words
+    uint32_t K;        // keys = num_vlabels * num_elabels
+    uint32_t EL;       // edge labels
+    const uint32_t *seg;     // n * (K + 1)
+    const uint32_t *nbr;     // partitioned adjacency
+};
+
+// Host-written description of one pattern in one eval call.
+struct PatStatic {
+    uint32_t nv, nslots, freq, mode;
+    uint32_t vlab[kMaxV];
+    uint32_t deg[kMaxV];
+    uint32_t nbr_n[kMaxV];
+    uint8_t nbr_v[kMaxV][kMaxV];        // pattern neighbours of u
+    uint32_t nbr_key[kMaxV][kMaxV];     // segment key of the edge u - nbr_v
+    uint32_t nlf_n[kMaxV];
+    uint32_t nlf_lab[kMaxV][kMaxV];
+    uint32_t nlf_cnt[kMaxV][kMaxV];
+    // LDF sources: parent domain (with its invalid bits) or label bucket
+    const uint32_t *src[kMaxV];
+    const uint32_t *src_invalid[kMaxV];
+    uint32_t src_n[kMaxV];
+    // domains
+    uint32_t *bits[kMaxV];
+    uint32_t *rank[kMaxV];
+    uint32_t *dom[kMaxV];
+    uint32_t dom_size[kMaxV];           // filled by the host once known
+    uint32_t *valid_bits[kMaxV];
+    uint32_t *invalid_bits[kMaxV];
+    // directed pattern edges
+    uint8_t slot_a[kMaxSlots], slot_b[kMaxSlots];
+    uint32_t slot_key[kMaxSlots];
+    uint32_t *aux_off[kMaxSlots];       // |C(a)| + 1 relative offsets
+    // matching plans per start vertex u: order[u][d], bn_slot[u][d][k]
+    uint8_t order[kMaxV][kMaxV];
+    uint8_t bn_n[kMaxV][kMaxV];
+    uint8_t bn_slot[kMaxV][kMaxV][kMaxV];
+    uint32_t task_prefix[kMaxV + 1];    // reference task order: u-major prefix of |C(u)|
+    const uint32_t *perm;               // optional task permutation (learned order)
+};
+
+// Device-written state of one pattern.
+struct PatState {
+    uint32_t dom_size[kMaxV];
+    uint32_t valid_cnt[kMaxV];
+    uint32_t invalid_cnt[kMaxV];
+    uint32_t ndone;
+    uint32_t status;                    // 0 undecided, 1 frequent, 2 infrequent
+    unsigned long long searches;
+    uint32_t aux_base[kMaxSlots];       // table start in the pool
+    uint32_t aux_total[kMaxSlots];
+};
+
+// Flattened work over (pattern, x, item) ranges: item t belongs to range m
+// with base[m] <= t < base[m+1].
+struct WorkTable {
+    const unsigned long long *base;     // M + 1
+    const uint32_t *pat;                // M
+    const uint32_t *x;                  // M (pattern vertex or slot)
+    uint32_t M;
+    unsigned long long total;
+};
+
+struct VerifyArgs {
+    const PatStatic *ps;
+    PatState *st;
+    const unsigned long long *task_base;   // P + 1
+    uint32_t P;
+    unsigned long long total;
+    unsigned long long *counter;
+    const uint32_t *pool;
+    uint32_t *trace;                       // nullable, indexed by reference task order
+    uint32_t steps_per_round;
+};
+
+}  // namespace tfsm_dev

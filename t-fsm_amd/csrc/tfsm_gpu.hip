@@ -1,0 +1,899 @@
+// libtfsm_gpu.so: the C ABI of include/tfsm_gpu.h on MI355X (gfx950).
+//
+// One translation unit: the kernels (kernels.inc) and the host side that
+// plans, launches and reads back an evaluation.  The host side follows one
+// pattern (tfsm_eval_pattern) or one extension level (tfsm_eval_level)
+// through the same launch sequence:
+//   LDF -> DP-iso passes until a pass removes nothing -> compaction
+//   -> (host) sizes, express / filter decisions, matching plans
+//   -> table count -> per-slot scan + pool allocation -> table fill
+//   -> existence search -> (host) verdicts and retained child domains.
+#include "kernels.inc"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+using namespace tfsm_dev;
+
+namespace {
+
+struct HipError {
+    hipError_t err;
+    const char *what;
+};
+
+#define HIPX(expr)                                              \
+    do {                                                        \
+        hipError_t e_ = (expr);                                 \
+        if (e_ != hipSuccess) throw HipError{e_, #expr};        \
+    } while (0)
+
+struct ArgError {
+    int code;
+    std::string msg;
+};
+
+// Chunked device arena: bump allocation, chunks never move, reset() reuses
+// every chunk.  Holds the scratch of one eval call.
+class Arena {
+public:
+    ~Arena() {
+        for (auto &c : chunks_) (void)hipFree(c.ptr);
+    }
+    void reset() {
+        for (auto &c : chunks_) c.used = 0;
+    }
+    template <class T>
+    T *alloc(size_t count) {
+        size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
+        if (bytes == 0) bytes = 256;
+        for (auto &c : chunks_) {
+            if (c.cap - c.used >= bytes) {
+                T *p = reinterpret_cast<T *>(c.ptr + c.used);
+                c.used += bytes;
+                return p;
+            }
+        }
+        size_t cap = std::max(bytes, size_t(256) << 20);
+        char *ptr = nullptr;
+        HIPX(hipMalloc(&ptr, cap));
+        chunks_.push_back({ptr, cap, bytes});
+        return reinterpret_cast<T *>(ptr);
+    }
+
+private:
+    struct Chunk {
+        char *ptr;
+        size_t cap, used;
+    };
+    std::vector<Chunk> chunks_;
+};
+
+// Size-class cache of device buffers for retained domains.
+class BufferCache {
+public:
+    ~BufferCache() {
+        for (auto &kv : free_) (void)hipFree(kv.second);
+    }
+    void *get(size_t bytes, size_t *cls) {
+        size_t c = 4096;
+        while (c < bytes) c <<= 1;
+        *cls = c;
+        auto it = free_.find(c);
+        if (it != free_.end()) {
+            void *p = it->second;
+            free_.erase(it);
+            return p;
+        }
+        void *p = nullptr;
+        HIPX(hipMalloc(&p, c));
+        return p;
+    }
+    void put(void *p, size_t cls) { free_.emplace(cls, p); }
+
+private:
+    std::multimap<size_t, void *> free_;
+};
+
+struct Retained {
+    void *buf = nullptr;
+    size_t cls = 0;
+    uint32_t nv = 0;
+    uint32_t size[kMaxV];
+    uint32_t *dom[kMaxV];           // device pointers into buf
+    uint32_t *invalid[kMaxV];
+};
+
+template <class T>
+struct Pinned {
+    T *ptr = nullptr;
+    size_t cap = 0;
+    ~Pinned() { if (ptr) (void)hipHostFree(ptr); }
+    T *ensure(size_t n) {
+        if (n > cap) {
+            if (ptr) HIPX(hipHostFree(ptr));
+            cap = std::max(n, cap * 2);
+            HIPX(hipHostMalloc(reinterpret_cast<void **>(&ptr), cap * sizeof(T), hipHostMallocDefault));
+        }
+        return ptr;
+    }
+};
+
+}  // namespace
+
+struct tfsm_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    // graph
+    uint32_t n = 0, L = 0, EL = 0, K = 0, W = 0;
+    uint64_t nadj = 0;
+    uint32_t *d_seg = nullptr, *d_nbr = nullptr, *d_bucket = nullptr;
+    std::vector<uint32_t> bucket_off;        // host copy, L + 1
+    DevGraph g{};
+    // scratch
+    Arena work;
+    uint32_t *d_pool = nullptr;
+    uint32_t pool_cap = 0;
+    uint32_t *d_flags = nullptr;             // [0] pool_used, [1] overflow, [2..] prune changed flags
+    unsigned long long *d_counter = nullptr;
+    Pinned<uint32_t> h_flags;
+    Pinned<PatState> h_state;
+    // retained domains
+    BufferCache cache;
+    std::unordered_map<uint64_t, Retained> retained;
+    uint64_t next_handle = 1;
+    // last call
+    std::vector<PatStatic> last_static;
+    std::vector<PatState> last_state;
+    std::vector<int> last_decided;
+    uint32_t *d_trace = nullptr;
+    uint64_t trace_len = 0, trace_cap = 0;
+    unsigned long long *d_scanned = nullptr;
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    tfsm_call_stats stats{};
+    std::string err;
+};
+
+namespace {
+
+int fail(tfsm_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+template <class F>
+int guarded(tfsm_ctx *c, F &&f) {
+    try {
+        return f();
+    } catch (const HipError &e) {
+        return fail(c, TFSM_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.err));
+    } catch (const ArgError &e) {
+        return fail(c, e.code, e.msg);
+    } catch (const std::bad_alloc &) {
+        return fail(c, TFSM_ERR_LIMIT, "host allocation failed");
+    }
+}
+
+uint32_t grid_for(unsigned long long items, int num_cus) {
+    unsigned long long blocks = (items + kBlock - 1) / kBlock;
+    unsigned long long cap = (unsigned long long)num_cus * 16;
+    return (uint32_t)std::max<unsigned long long>(1, std::min(blocks, cap));
+}
+
+template <class T>
+T *upload(tfsm_ctx *c, const std::vector<T> &v) {
+    T *d = c->work.alloc<T>(std::max<size_t>(v.size(), 1));
+    if (!v.empty()) HIPX(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return d;
+}
+
+struct HostWork {
+    std::vector<unsigned long long> base{0};
+    std::vector<uint32_t> pat, x;
+    void add(uint32_t p, uint32_t xx, uint64_t count) {
+        if (count == 0) return;
+        pat.push_back(p);
+        x.push_back(xx);
+        base.push_back(base.back() + count);
+    }
+    WorkTable to_device(tfsm_ctx *c) const {
+        WorkTable w;
+        w.base = upload(c, base);
+        w.pat = upload(c, pat);
+        w.x = upload(c, x);
+        w.M = (uint32_t)pat.size();
+        w.total = base.back();
+        return w;
+    }
+};
+
+// Pattern checks: sizes, labels, edge endpoints, no duplicate edges, connected.
+void check_pattern(const tfsm_ctx *c, const tfsm_pattern &p) {
+    if (p.num_vertices < 2 || p.num_vertices > kMaxV)
+        throw ArgError{TFSM_ERR_LIMIT, "pattern vertex count out of range [2, 16]"};
+    if (p.num_edges > kMaxE) throw ArgError{TFSM_ERR_LIMIT, "pattern has more than 64 edges"};
+    if (p.num_edges < p.num_vertices - 1) throw ArgError{TFSM_ERR_ARGUMENT, "pattern is not connected"};
+    for (uint32_t i = 0; i < p.num_vertices; ++i)
+        if (p.vlabel[i] >= c->L) throw ArgError{TFSM_ERR_ARGUMENT, "pattern vertex label out of range"};
+    uint32_t adj[kMaxV] = {0};
+    for (uint32_t e = 0; e < p.num_edges; ++e) {
+        uint32_t a = p.edge_a[e], b = p.edge_b[e];
+        if (a >= p.num_vertices || b >= p.num_vertices || a == b)
+            throw ArgError{TFSM_ERR_ARGUMENT, "pattern edge endpoint out of range"};
+        if (p.edge_label[e] >= c->EL) throw ArgError{TFSM_ERR_ARGUMENT, "pattern edge label out of range"};
+        if (adj[a] >> b & 1) throw ArgError{TFSM_ERR_ARGUMENT, "duplicate pattern edge"};
+        adj[a] |= 1u << b;
+        adj[b] |= 1u << a;
+    }
+    uint32_t seen = 1, frontier = 1;
+    while (frontier) {
+        uint32_t nxt = 0;
+        for (uint32_t i = 0; i < p.num_vertices; ++i)
+            if (frontier >> i & 1) nxt |= adj[i];
+        frontier = nxt & ~seen;
+        seen |= nxt;
+    }
+    if (seen != (1u << p.num_vertices) - 1) throw ArgError{TFSM_ERR_ARGUMENT, "pattern is not connected"};
+}
+
+// GQL order and backward neighbours per start vertex (reference
+// generateGQLQueryPlan(int, ui*&) and generateBN(...), gmatch.h:692-727,
+// 803-828): next = the unvisited vertex adjacent to the visited set with the
+// fewest candidates, ties to the higher pattern degree, then the lower id;
+// bn lists follow the pattern's ascending adjacency.
+void make_plans(PatStatic &P, const uint32_t *slot_of /* [kMaxV*kMaxV] */) {
+    const uint32_t nv = P.nv;
+    for (uint32_t start = 0; start < nv; ++start) {
+        bool visited[kMaxV] = {false}, adjacent[kMaxV] = {false};
+        uint8_t order[kMaxV];
+        auto visit = [&](uint32_t x) {
+            visited[x] = true;
+            for (uint32_t j = 0; j < P.nbr_n[x]; ++j) adjacent[P.nbr_v[x][j]] = true;
+        };
+        order[0] = (uint8_t)start;
+        visit(start);
+        for (uint32_t i = 1; i < nv; ++i) {
+            int next = -1;
+            for (uint32_t j = 0; j < nv; ++j) {
+                if (visited[j] || !adjacent[j]) continue;
+                if (next < 0 || P.dom_size[j] < P.dom_size[next] ||
+                    (P.dom_size[j] == P.dom_size[next] && P.deg[j] > P.deg[next]))
+                    next = (int)j;
+            }
+            order[i] = (uint8_t)next;
+            visit((uint32_t)next);
+        }
+        bool vis[kMaxV] = {false};
+        vis[order[0]] = true;
+        for (uint32_t d = 0; d < nv; ++d) {
+            P.order[start][d] = order[d];
+            P.bn_n[start][d] = 0;
+            if (d == 0) continue;
+            const uint32_t x = order[d];
+            for (uint32_t j = 0; j < P.nbr_n[x]; ++j) {
+                const uint32_t y = P.nbr_v[x][j];
+                if (vis[y]) P.bn_slot[start][d][P.bn_n[start][d]++] = (uint8_t)slot_of[y * kMaxV + x];
+            }
+            vis[x] = true;
+        }
+    }
+}
+
+struct EvalCall {
+    const tfsm_pattern *ps;
+    const uint64_t *parents;
+    uint32_t n;
+    uint32_t freq;
+    tfsm_eval_options opt;
+    tfsm_pattern_result *out;
+};
+
+int run_eval(tfsm_ctx *c, const EvalCall &call) {
+    const uint32_t n = call.n;
+    const tfsm_eval_options &opt = call.opt;
+    if (opt.mode != TFSM_MODE_CAPPED && opt.mode != TFSM_MODE_EXACT)
+        throw ArgError{TFSM_ERR_ARGUMENT, "unknown mode"};
+    if (opt.task_order != TFSM_ORDER_REFERENCE)
+        throw ArgError{TFSM_ERR_ARGUMENT, "learned task order requires an installed ranker"};
+    if (call.freq == 0) throw ArgError{TFSM_ERR_ARGUMENT, "freq must be at least 1"};
+    for (uint32_t i = 0; i < n; ++i) {
+        check_pattern(c, call.ps[i]);
+        uint64_t h = call.parents ? call.parents[i] : 0;
+        if (h) {
+            auto it = c->retained.find(h);
+            if (it == c->retained.end()) throw ArgError{TFSM_ERR_HANDLE, "unknown parent domain handle"};
+            if (it->second.nv > call.ps[i].num_vertices)
+                throw ArgError{TFSM_ERR_ARGUMENT, "parent has more vertices than the pattern"};
+        }
+    }
+    c->work.reset();
+    hipStream_t s = c->stream;
+    HIPX(hipEventRecord(c->ev[0], s));
+
+    // ---------------- host description of every pattern ----------------
+    std::vector<PatStatic> hs(n);
+    std::vector<uint32_t> slot_of((size_t)n * kMaxV * kMaxV, 0xffffffffu);
+    std::vector<char> distinct(n), acyclic(n);
+    size_t bits_words = 0, dom_words = 0;
+    for (uint32_t p = 0; p < n; ++p) {
+        const tfsm_pattern &tp = call.ps[p];
+        PatStatic &P = hs[p];
+        std::memset(&P, 0, sizeof(P));
+        P.nv = tp.num_vertices;
+        P.freq = call.freq;
+        P.mode = opt.mode;
+        for (uint32_t i = 0; i < P.nv; ++i) P.vlab[i] = tp.vlabel[i];
+        uint32_t el_of[kMaxV][kMaxV];
+        bool adj[kMaxV][kMaxV] = {{false}};
+        for (uint32_t e = 0; e < tp.num_edges; ++e) {
+            const uint32_t a = tp.edge_a[e], b = tp.edge_b[e];
+            adj[a][b] = adj[b][a] = true;
+            el_of[a][b] = el_of[b][a] = tp.edge_label[e];
+            uint32_t s0 = P.nslots++, s1 = P.nslots++;
+            P.slot_a[s0] = (uint8_t)a; P.slot_b[s0] = (uint8_t)b;
+            P.slot_key[s0] = P.vlab[b] * c->EL + tp.edge_label[e];
+            P.slot_a[s1] = (uint8_t)b; P.slot_b[s1] = (uint8_t)a;
+            P.slot_key[s1] = P.vlab[a] * c->EL + tp.edge_label[e];
+            slot_of[(size_t)p * kMaxV * kMaxV + a * kMaxV + b] = s0;
+            slot_of[(size_t)p * kMaxV * kMaxV + b * kMaxV + a] = s1;
+        }
+        for (uint32_t a = 0; a < P.nv; ++a) {
+            uint32_t cnt_by_label[kMaxV], lab_of[kMaxV], nl = 0;
+            for (uint32_t b = 0; b < P.nv; ++b) {
+                if (!adj[a][b]) continue;
+                P.nbr_v[a][P.nbr_n[a]] = (uint8_t)b;
+                P.nbr_key[a][P.nbr_n[a]] = P.vlab[b] * c->EL + el_of[a][b];
+                P.nbr_n[a]++;
+                uint32_t k = 0;
+                while (k < nl && lab_of[k] != P.vlab[b]) ++k;
+                if (k == nl) { lab_of[nl] = P.vlab[b]; cnt_by_label[nl] = 0; ++nl; }
+                cnt_by_label[k]++;
+            }
+            P.deg[a] = P.nbr_n[a];
+            P.nlf_n[a] = nl;
+            for (uint32_t k = 0; k < nl; ++k) { P.nlf_lab[a][k] = lab_of[k]; P.nlf_cnt[a][k] = cnt_by_label[k]; }
+        }
+        const uint64_t h = call.parents ? call.parents[p] : 0;
+        const Retained *par = h ? &c->retained.at(h) : nullptr;
+        for (uint32_t u = 0; u < P.nv; ++u) {
+            if (par && u < par->nv) {
+                P.src[u] = par->dom[u];
+                P.src_invalid[u] = par->invalid[u];
+                P.src_n[u] = par->size[u];
+            } else {
+                P.src[u] = c->d_bucket + c->bucket_off[P.vlab[u]];
+                P.src_invalid[u] = nullptr;
+                P.src_n[u] = c->bucket_off[P.vlab[u] + 1] - c->bucket_off[P.vlab[u]];
+            }
+            bits_words += 2 * (size_t)c->W;
+            dom_words += std::max<uint32_t>(P.src_n[u], 1);
+        }
+        bool dl = true;
+        for (uint32_t a = 0; a < P.nv; ++a)
+            for (uint32_t b = a + 1; b < P.nv; ++b) if (P.vlab[a] == P.vlab[b]) dl = false;
+        distinct[p] = dl;
+        acyclic[p] = tp.num_edges == tp.num_vertices - 1;
+    }
+    // one region for all bitmaps (zeroed at once) and one for all lists
+    uint32_t *bits_region = c->work.alloc<uint32_t>(bits_words);
+    uint32_t *dom_region = c->work.alloc<uint32_t>(dom_words);
+    HIPX(hipMemsetAsync(bits_region, 0, bits_words * sizeof(uint32_t), s));
+    {
+        size_t bo = 0, dofs = 0;
+        for (uint32_t p = 0; p < n; ++p)
+            for (uint32_t u = 0; u < hs[p].nv; ++u) {
+                hs[p].bits[u] = bits_region + bo; bo += c->W;
+                hs[p].rank[u] = bits_region + bo; bo += c->W;
+                hs[p].dom[u] = dom_region + dofs; dofs += std::max<uint32_t>(hs[p].src_n[u], 1);
+            }
+    }
+    PatStatic *d_ps = c->work.alloc<PatStatic>(n);
+    PatState *d_st = c->work.alloc<PatState>(n);
+    HIPX(hipMemcpyAsync(d_ps, hs.data(), n * sizeof(PatStatic), hipMemcpyHostToDevice, s));
+    HIPX(hipMemsetAsync(d_st, 0, n * sizeof(PatState), s));
+
+    // ---------------- filter: LDF, DP-iso to the fixpoint, compaction ----------------
+    HostWork src_work;
+    std::vector<uint32_t> pair_p, pair_u;
+    for (uint32_t p = 0; p < n; ++p)
+        for (uint32_t u = 0; u < hs[p].nv; ++u) {
+            src_work.add(p, u, hs[p].src_n[u]);
+            pair_p.push_back(p);
+            pair_u.push_back(u);
+        }
+    WorkTable wsrc = src_work.to_device(c);
+    const uint32_t *d_pair_p = upload(c, pair_p), *d_pair_u = upload(c, pair_u);
+    const uint32_t gsrc = grid_for(wsrc.total, c->num_cus);
+    if (wsrc.total) hipLaunchKernelGGL(k_ldf, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc);
+    HIPX(hipGetLastError());
+    uint32_t *hf = c->h_flags.ensure(8);
+    for (int round = 0; wsrc.total; ++round) {
+        HIPX(hipMemsetAsync(c->d_flags + 2, 0, 2 * sizeof(uint32_t), s));
+        hipLaunchKernelGGL(k_prune, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc, c->d_flags + 2);
+        hipLaunchKernelGGL(k_prune, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc, c->d_flags + 3);
+        HIPX(hipGetLastError());
+        HIPX(hipMemcpyAsync(hf, c->d_flags + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPX(hipStreamSynchronize(s));
+        if (hf[0] == 0) break;
+        if (round > 100000) throw ArgError{TFSM_ERR_LIMIT, "filter did not converge"};
+    }
+    hipLaunchKernelGGL(k_compact, dim3((uint32_t)pair_p.size()), dim3(kBlock), 0, s, c->g, d_ps, d_st, d_pair_p, d_pair_u);
+    HIPX(hipGetLastError());
+    PatState *hst = c->h_state.ensure(n);
+    HIPX(hipMemcpyAsync(hst, d_st, n * sizeof(PatState), hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    HIPX(hipEventRecord(c->ev[1], s));
+
+    // ---------------- decisions and plans ----------------
+    std::vector<int> decided(n);
+    size_t aux_words = 0, verdict_words = 0;
+    for (uint32_t p = 0; p < n; ++p) {
+        PatStatic &P = hs[p];
+        uint32_t mn = 0xffffffffu;
+        for (uint32_t u = 0; u < P.nv; ++u) {
+            P.dom_size[u] = hst[p].dom_size[u];
+            mn = std::min(mn, P.dom_size[u]);
+        }
+        if (distinct[p] && acyclic[p]) decided[p] = TFSM_DECIDED_EXPRESS;     // comper.h:1619-1653
+        else if (mn < call.freq) decided[p] = TFSM_DECIDED_FILTER;            // isCandidateSetValid(freq)
+        else decided[p] = TFSM_DECIDED_SEARCH;
+        if (decided[p] != TFSM_DECIDED_SEARCH) continue;
+        make_plans(P, slot_of.data() + (size_t)p * kMaxV * kMaxV);
+        P.task_prefix[0] = 0;
+        for (uint32_t u = 0; u < P.nv; ++u) {
+            P.task_prefix[u + 1] = P.task_prefix[u] + P.dom_size[u];
+            verdict_words += 2 * (size_t)((P.dom_size[u] + 31) / 32);
+        }
+        for (uint32_t sl = 0; sl < P.nslots; ++sl) aux_words += P.dom_size[P.slot_a[sl]] + 1;
+    }
+    uint32_t *aux_region = c->work.alloc<uint32_t>(std::max<size_t>(aux_words, 1));
+    uint32_t *verdict_region = c->work.alloc<uint32_t>(std::max<size_t>(verdict_words, 1));
+    if (verdict_words) HIPX(hipMemsetAsync(verdict_region, 0, verdict_words * sizeof(uint32_t), s));
+    HostWork aux_work;
+    std::vector<uint32_t> slot_p, slot_s;
+    std::vector<unsigned long long> task_base{0};
+    {
+        size_t ao = 0, vo = 0;
+        for (uint32_t p = 0; p < n; ++p) {
+            PatStatic &P = hs[p];
+            unsigned long long ntask = 0;
+            if (decided[p] == TFSM_DECIDED_SEARCH) {
+                for (uint32_t sl = 0; sl < P.nslots; ++sl) {
+                    P.aux_off[sl] = aux_region + ao;
+                    ao += P.dom_size[P.slot_a[sl]] + 1;
+                    aux_work.add(p, sl, P.dom_size[P.slot_a[sl]]);
+                    slot_p.push_back(p);
+                    slot_s.push_back(sl);
+                }
+                for (uint32_t u = 0; u < P.nv; ++u) {
+                    const size_t wds = (P.dom_size[u] + 31) / 32;
+                    P.valid_bits[u] = verdict_region + vo; vo += wds;
+                    P.invalid_bits[u] = verdict_region + vo; vo += wds;
+                }
+                ntask = P.task_prefix[P.nv];
+            }
+            task_base.push_back(task_base.back() + ntask);
+        }
+    }
+    HIPX(hipMemcpyAsync(d_ps, hs.data(), n * sizeof(PatStatic), hipMemcpyHostToDevice, s));
+
+    // ---------------- auxiliary tables ----------------
+    if (!slot_p.empty()) {
+        WorkTable waux = aux_work.to_device(c);
+        const uint32_t *d_slot_p = upload(c, slot_p), *d_slot_s = upload(c, slot_s);
+        const uint32_t gaux = grid_for(waux.total, c->num_cus);
+        for (int attempt = 0;; ++attempt) {
+            HIPX(hipMemsetAsync(c->d_flags, 0, 2 * sizeof(uint32_t), s));
+            HIPX(hipMemsetAsync(c->d_scanned, 0, sizeof(unsigned long long), s));
+            if (waux.total)
+                hipLaunchKernelGGL(k_aux_count, dim3(gaux), dim3(kBlock), 0, s, c->g, d_ps, waux, c->d_scanned);
+            hipLaunchKernelGGL(k_aux_scan, dim3((uint32_t)slot_p.size()), dim3(kBlock), 0, s, d_ps, d_st, d_slot_p,
+                               d_slot_s, c->d_flags, c->pool_cap, c->d_flags + 1);
+            if (waux.total)
+                hipLaunchKernelGGL(k_aux_fill, dim3(gaux), dim3(kBlock), 0, s, c->g, d_ps, d_st, waux, c->d_pool,
+                                   c->d_flags + 1);
+            HIPX(hipGetLastError());
+            HIPX(hipMemcpyAsync(hf, c->d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIPX(hipStreamSynchronize(s));
+            if (!hf[1]) break;
+            if (attempt > 4) throw ArgError{TFSM_ERR_LIMIT, "table pool overflow"};
+            // grow the pool to the measured need and rebuild
+            unsigned long long need = (unsigned long long)hf[0] * 2;
+            if (need > 0xfffffff0ull) throw ArgError{TFSM_ERR_LIMIT, "tables exceed 2^32 entries"};
+            HIPX(hipFree(c->d_pool));
+            c->d_pool = nullptr;
+            c->pool_cap = (uint32_t)need;
+            HIPX(hipMalloc(&c->d_pool, (size_t)c->pool_cap * sizeof(uint32_t)));
+        }
+    }
+    unsigned long long scanned = 0;
+    if (!slot_p.empty()) HIPX(hipMemcpy(&scanned, c->d_scanned, sizeof(scanned), hipMemcpyDeviceToHost));
+    HIPX(hipEventRecord(c->ev[2], s));
+
+    // ---------------- existence search ----------------
+    const unsigned long long total_tasks = task_base.back();
+    c->trace_len = 0;
+    if (opt.trace) {
+        if (total_tasks > c->trace_cap) {
+            if (c->d_trace) HIPX(hipFree(c->d_trace));
+            c->d_trace = nullptr;
+            c->trace_cap = std::max<unsigned long long>(total_tasks, 1 << 20);
+            HIPX(hipMalloc(&c->d_trace, c->trace_cap * sizeof(uint32_t)));
+        }
+        c->trace_len = total_tasks;
+        if (total_tasks) HIPX(hipMemsetAsync(c->d_trace, 0, total_tasks * sizeof(uint32_t), s));
+    }
+    if (total_tasks) {
+        VerifyArgs va;
+        va.ps = d_ps;
+        va.st = d_st;
+        va.task_base = upload(c, task_base);
+        va.P = n;
+        va.total = total_tasks;
+        va.counter = c->d_counter;
+        va.pool = c->d_pool;
+        va.trace = opt.trace ? c->d_trace : nullptr;
+        va.steps_per_round = 4;
+        HIPX(hipMemsetAsync(c->d_counter, 0, sizeof(unsigned long long), s));
+        HIPX(hipEventRecord(c->ev[4], s));
+        uint32_t max_nv = 0;
+        for (uint32_t p = 0; p < n; ++p) if (decided[p] == TFSM_DECIDED_SEARCH) max_nv = std::max(max_nv, hs[p].nv);
+        const uint32_t per_cu = opt.grid_waves_per_cu ? std::max(1u, opt.grid_waves_per_cu / (kBlock / 64)) : 4u;
+        unsigned long long want = (total_tasks + kBlock - 1) / kBlock;
+        uint32_t grid = (uint32_t)std::max<unsigned long long>(1, std::min<unsigned long long>(want, (unsigned long long)c->num_cus * per_cu));
+        if (max_nv <= 8) hipLaunchKernelGGL(k_verify<8>, dim3(grid), dim3(kBlock), 0, s, va);
+        else hipLaunchKernelGGL(k_verify<16>, dim3(grid), dim3(kBlock), 0, s, va);
+        HIPX(hipGetLastError());
+        HIPX(hipEventRecord(c->ev[5], s));
+    }
+    HIPX(hipEventRecord(c->ev[3], s));
+    HIPX(hipMemcpyAsync(hst, d_st, n * sizeof(PatState), hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    HIPX(hipEventElapsedTime(&c->stats.filter_ms, c->ev[0], c->ev[1]));
+    HIPX(hipEventElapsedTime(&c->stats.table_ms, c->ev[1], c->ev[2]));
+    if (total_tasks) {
+        HIPX(hipEventElapsedTime(&c->stats.search_ms, c->ev[4], c->ev[5]));
+        c->stats.search_launches = 1;
+    }
+    {
+        // algorithmic bytes (definitions in tfsm_gpu.h)
+        unsigned long long tb = 4ull * scanned, sb = 0;
+        for (uint32_t p = 0; p < n; ++p) {
+            if (decided[p] != TFSM_DECIDED_SEARCH) continue;
+            const PatStatic &P = hs[p];
+            const PatState &S = hst[p];
+            for (uint32_t sl = 0; sl < P.nslots; ++sl) {
+                const unsigned long long rows = P.dom_size[P.slot_a[sl]];
+                tb += 8ull * rows + 4ull * (rows + 1) + 4ull * S.aux_total[sl];
+            }
+            for (uint32_t u = 0; u < P.nv; ++u) {
+                if (!P.dom_size[u]) continue;
+                double per = 0;
+                for (uint32_t sl = 0; sl < P.nslots; ++sl)
+                    if (P.slot_a[sl] == u) per += 8.0 + 4.0 * (double)S.aux_total[sl] / (double)P.dom_size[u];
+                sb += (unsigned long long)(per * (double)(S.valid_cnt[u] + S.invalid_cnt[u]));
+            }
+        }
+        c->stats.table_bytes = tb;
+        c->stats.search_bytes = sb;
+    }
+
+    // ---------------- results and retained child domains ----------------
+    for (uint32_t p = 0; p < n; ++p) {
+        const PatStatic &P = hs[p];
+        const PatState &S = hst[p];
+        tfsm_pattern_result &r = call.out[p];
+        std::memset(&r, 0, sizeof(r));
+        r.decided_by = decided[p];
+        uint32_t lo = 0xffffffffu, hi = 0xffffffffu;
+        for (uint32_t u = 0; u < P.nv; ++u) {
+            r.domain_size[u] = P.dom_size[u];
+            if (decided[p] == TFSM_DECIDED_EXPRESS) {
+                r.valid_count[u] = P.dom_size[u];
+            } else if (decided[p] == TFSM_DECIDED_SEARCH) {
+                r.valid_count[u] = S.valid_cnt[u];
+                r.invalid_count[u] = S.invalid_cnt[u];
+            }
+            lo = std::min(lo, r.valid_count[u]);
+            hi = std::min(hi, r.domain_size[u] - r.invalid_count[u]);
+            r.verdicts += r.valid_count[u] + r.invalid_count[u];
+        }
+        r.support_lower = lo;
+        r.support_upper = hi;
+        r.searches = S.searches;
+        if (decided[p] == TFSM_DECIDED_SEARCH && opt.mode == TFSM_MODE_CAPPED && S.status != 0)
+            r.frequent = S.status == 1;
+        else
+            r.frequent = decided[p] != TFSM_DECIDED_FILTER && lo >= call.freq;
+        if (r.frequent && opt.retain_domains) {
+            size_t words = 0;
+            for (uint32_t u = 0; u < P.nv; ++u) words += P.dom_size[u] + (P.dom_size[u] + 31) / 32;
+            Retained R;
+            R.nv = P.nv;
+            R.buf = c->cache.get(std::max<size_t>(words, 1) * sizeof(uint32_t), &R.cls);
+            uint32_t *b = static_cast<uint32_t *>(R.buf);
+            for (uint32_t u = 0; u < P.nv; ++u) {
+                const size_t wds = (P.dom_size[u] + 31) / 32;
+                R.size[u] = P.dom_size[u];
+                R.dom[u] = b; b += P.dom_size[u];
+                R.invalid[u] = b; b += wds;
+                if (P.dom_size[u])
+                    HIPX(hipMemcpyAsync(R.dom[u], P.dom[u], P.dom_size[u] * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+                if (wds) {
+                    if (decided[p] == TFSM_DECIDED_SEARCH)
+                        HIPX(hipMemcpyAsync(R.invalid[u], P.invalid_bits[u], wds * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+                    else
+                        HIPX(hipMemsetAsync(R.invalid[u], 0, wds * sizeof(uint32_t), s));
+                }
+            }
+            r.domain_handle = c->next_handle++;
+            c->retained.emplace(r.domain_handle, R);
+        }
+    }
+    c->last_static = hs;
+    c->last_state.assign(hst, hst + n);
+    c->last_decided = decided;
+    HIPX(hipStreamSynchronize(s));
+    return TFSM_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+const char *tfsm_status_string(int status) {
+    switch (status) {
+        case TFSM_OK: return "ok";
+        case TFSM_ERR_ARGUMENT: return "invalid argument";
+        case TFSM_ERR_HIP: return "HIP runtime error";
+        case TFSM_ERR_NO_DEVICE: return "no usable GPU";
+        case TFSM_ERR_LIMIT: return "input beyond a supported limit";
+        case TFSM_ERR_HANDLE: return "unknown domain handle";
+        default: return "unknown status";
+    }
+}
+
+const char *tfsm_last_error(tfsm_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
+    if (!g || !out) return TFSM_ERR_ARGUMENT;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return TFSM_ERR_NO_DEVICE;
+    if (g->num_vertices == 0 || g->num_vlabels == 0 || g->num_elabels == 0) return TFSM_ERR_ARGUMENT;
+    if (!g->offsets || (g->num_adj && (!g->nbrs || !g->elabels)) || !g->vlabels) return TFSM_ERR_ARGUMENT;
+    if (g->num_adj >= 0xffffffffull) return TFSM_ERR_LIMIT;
+    const uint64_t K = (uint64_t)g->num_vlabels * g->num_elabels;
+    if (K > 4096) return TFSM_ERR_LIMIT;
+    auto *c = new tfsm_ctx;
+    int rc = guarded(c, [&]() -> int {
+        HIPX(hipGetDevice(&c->device));
+        hipDeviceProp_t prop;
+        HIPX(hipGetDeviceProperties(&prop, c->device));
+        c->num_cus = prop.multiProcessorCount;
+        HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto &e : c->ev) HIPX(hipEventCreate(&e));
+        c->n = g->num_vertices;
+        c->L = g->num_vlabels;
+        c->EL = g->num_elabels;
+        c->K = (uint32_t)K;
+        c->W = (c->n + 31) / 32;
+        c->nadj = g->num_adj;
+        // validate and partition the adjacency by (label(w), edge label), keeping id order
+        if (g->offsets[0] != 0 || g->offsets[c->n] != g->num_adj) throw ArgError{TFSM_ERR_ARGUMENT, "bad offsets"};
+        for (uint32_t v = 0; v < c->n; ++v) {
+            if (g->offsets[v + 1] < g->offsets[v]) throw ArgError{TFSM_ERR_ARGUMENT, "offsets not monotone"};
+            if (g->vlabels[v] >= c->L) throw ArgError{TFSM_ERR_ARGUMENT, "vertex label out of range"};
+        }
+        std::vector<uint32_t> seg((size_t)c->n * (c->K + 1));
+        std::vector<uint32_t> nbr(g->num_adj);
+        std::vector<uint32_t> cnt(c->K + 1);
+        for (uint32_t v = 0; v < c->n; ++v) {
+            const uint32_t b = g->offsets[v], e = g->offsets[v + 1];
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (uint32_t k = b; k < e; ++k) {
+                const uint32_t w = g->nbrs[k];
+                if (w >= c->n || g->elabels[k] >= c->EL) throw ArgError{TFSM_ERR_ARGUMENT, "adjacency entry out of range"};
+                if (k > b && g->nbrs[k - 1] > w) throw ArgError{TFSM_ERR_ARGUMENT, "adjacency not ascending"};
+                cnt[g->vlabels[w] * c->EL + g->elabels[k] + 1]++;
+            }
+            uint32_t *sv = &seg[(size_t)v * (c->K + 1)];
+            uint32_t acc = b;
+            for (uint32_t k = 0; k < c->K; ++k) {
+                sv[k] = acc;
+                acc += cnt[k + 1];
+            }
+            sv[c->K] = acc;
+            std::vector<uint32_t> fill(sv, sv + c->K);
+            for (uint32_t k = b; k < e; ++k) {
+                const uint32_t w = g->nbrs[k];
+                nbr[fill[g->vlabels[w] * c->EL + g->elabels[k]]++] = w;
+            }
+        }
+        std::vector<uint32_t> bucket(c->n);
+        c->bucket_off.assign(c->L + 1, 0);
+        for (uint32_t v = 0; v < c->n; ++v) c->bucket_off[g->vlabels[v] + 1]++;
+        for (uint32_t l = 0; l < c->L; ++l) c->bucket_off[l + 1] += c->bucket_off[l];
+        {
+            std::vector<uint32_t> f(c->bucket_off.begin(), c->bucket_off.end() - 1);
+            for (uint32_t v = 0; v < c->n; ++v) bucket[f[g->vlabels[v]]++] = v;
+        }
+        HIPX(hipMalloc(&c->d_seg, seg.size() * sizeof(uint32_t)));
+        HIPX(hipMalloc(&c->d_nbr, std::max<size_t>(nbr.size(), 1) * sizeof(uint32_t)));
+        HIPX(hipMalloc(&c->d_bucket, bucket.size() * sizeof(uint32_t)));
+        HIPX(hipMemcpy(c->d_seg, seg.data(), seg.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        if (!nbr.empty()) HIPX(hipMemcpy(c->d_nbr, nbr.data(), nbr.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIPX(hipMemcpy(c->d_bucket, bucket.data(), bucket.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        c->g.n = c->n;
+        c->g.W = c->W;
+        c->g.K = c->K;
+        c->g.EL = c->EL;
+        c->g.seg = c->d_seg;
+        c->g.nbr = c->d_nbr;
+        c->pool_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(4 * g->num_adj, 1u << 22), 0xfffffff0ull);
+        HIPX(hipMalloc(&c->d_pool, (size_t)c->pool_cap * sizeof(uint32_t)));
+        HIPX(hipMalloc(&c->d_flags, 16 * sizeof(uint32_t)));
+        HIPX(hipMalloc(&c->d_counter, sizeof(unsigned long long)));
+        HIPX(hipMalloc(&c->d_scanned, sizeof(unsigned long long)));
+        return TFSM_OK;
+    });
+    if (rc != TFSM_OK) {
+        tfsm_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return TFSM_OK;
+}
+
+int tfsm_ctx_destroy(tfsm_ctx *c) {
+    if (!c) return TFSM_ERR_ARGUMENT;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &kv : c->retained) c->cache.put(kv.second.buf, kv.second.cls);
+    c->retained.clear();
+    (void)hipFree(c->d_seg);
+    (void)hipFree(c->d_nbr);
+    (void)hipFree(c->d_bucket);
+    (void)hipFree(c->d_pool);
+    (void)hipFree(c->d_flags);
+    (void)hipFree(c->d_counter);
+    (void)hipFree(c->d_scanned);
+    (void)hipFree(c->d_trace);
+    for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TFSM_OK;
+}
+
+int tfsm_eval_pattern(tfsm_ctx *ctx, const tfsm_pattern *p, uint64_t parent_handle, uint32_t freq,
+                      const tfsm_eval_options *opt, tfsm_pattern_result *out) {
+    if (!ctx || !p || !opt || !out) return TFSM_ERR_ARGUMENT;
+    return guarded(ctx, [&]() { return run_eval(ctx, EvalCall{p, &parent_handle, 1, freq, *opt, out}); });
+}
+
+int tfsm_eval_level(tfsm_ctx *ctx, const tfsm_pattern *ps, const uint64_t *parent_handles, uint32_t n, uint32_t freq,
+                    const tfsm_eval_options *opt, tfsm_pattern_result *outs) {
+    if (!ctx || !opt || (n && (!ps || !outs))) return TFSM_ERR_ARGUMENT;
+    if (n == 0) return TFSM_OK;
+    return guarded(ctx, [&]() { return run_eval(ctx, EvalCall{ps, parent_handles, n, freq, *opt, outs}); });
+}
+
+int tfsm_domain_read(tfsm_ctx *ctx, uint64_t handle, uint32_t u, uint32_t *out, uint32_t cap, uint32_t *size) {
+    if (!ctx) return TFSM_ERR_ARGUMENT;
+    auto it = ctx->retained.find(handle);
+    if (it == ctx->retained.end()) return fail(ctx, TFSM_ERR_HANDLE, "unknown domain handle");
+    const Retained &R = it->second;
+    if (u >= R.nv) return fail(ctx, TFSM_ERR_ARGUMENT, "pattern vertex out of range");
+    return guarded(ctx, [&]() -> int {
+        const uint32_t n = R.size[u];
+        std::vector<uint32_t> dom(n), inv((n + 31) / 32);
+        if (n) {
+            HIPX(hipMemcpy(dom.data(), R.dom[u], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            HIPX(hipMemcpy(inv.data(), R.invalid[u], inv.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        }
+        uint32_t k = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (inv[i >> 5] >> (i & 31) & 1) continue;
+            if (out && k < cap) out[k] = dom[i];
+            ++k;
+        }
+        if (size) *size = k;
+        return TFSM_OK;
+    });
+}
+
+int tfsm_domain_release(tfsm_ctx *ctx, uint64_t handle) {
+    if (!ctx) return TFSM_ERR_ARGUMENT;
+    auto it = ctx->retained.find(handle);
+    if (it == ctx->retained.end()) return fail(ctx, TFSM_ERR_HANDLE, "unknown domain handle");
+    ctx->cache.put(it->second.buf, it->second.cls);
+    ctx->retained.erase(it);
+    return TFSM_OK;
+}
+
+int64_t tfsm_last_domain(tfsm_ctx *ctx, uint32_t u, uint32_t *out, uint32_t cap) {
+    if (!ctx || ctx->last_static.size() != 1 || u >= ctx->last_static[0].nv) return TFSM_ERR_ARGUMENT;
+    const PatStatic &P = ctx->last_static[0];
+    const uint32_t n = P.dom_size[u];
+    int rc = guarded(ctx, [&]() -> int {
+        if (out && n) HIPX(hipMemcpy(out, P.dom[u], std::min(n, cap) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        return TFSM_OK;
+    });
+    return rc == TFSM_OK ? (int64_t)n : rc;
+}
+
+int64_t tfsm_last_states(tfsm_ctx *ctx, uint32_t u, uint8_t *out, uint32_t cap) {
+    if (!ctx || ctx->last_static.size() != 1 || u >= ctx->last_static[0].nv) return TFSM_ERR_ARGUMENT;
+    const PatStatic &P = ctx->last_static[0];
+    const int decided = ctx->last_decided[0];
+    const uint32_t n = P.dom_size[u];
+    int rc = guarded(ctx, [&]() -> int {
+        std::vector<uint32_t> vb((n + 31) / 32, 0), ib((n + 31) / 32, 0);
+        if (decided == TFSM_DECIDED_SEARCH && n) {
+            HIPX(hipMemcpy(vb.data(), P.valid_bits[u], vb.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            HIPX(hipMemcpy(ib.data(), P.invalid_bits[u], ib.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        }
+        for (uint32_t i = 0; i < n && i < cap && out; ++i) {
+            if (decided == TFSM_DECIDED_EXPRESS) out[i] = 1;
+            else out[i] = (vb[i >> 5] >> (i & 31) & 1) ? 1 : ((ib[i >> 5] >> (i & 31) & 1) ? 2 : 0);
+        }
+        return TFSM_OK;
+    });
+    return rc == TFSM_OK ? (int64_t)n : rc;
+}
+
+int64_t tfsm_last_aux(tfsm_ctx *ctx, uint32_t a, uint32_t b, uint32_t *off, uint32_t off_cap, uint32_t *idx,
+                      uint32_t idx_cap) {
+    if (!ctx || ctx->last_static.size() != 1) return TFSM_ERR_ARGUMENT;
+    const PatStatic &P = ctx->last_static[0];
+    if (a >= P.nv || b >= P.nv || ctx->last_decided[0] != TFSM_DECIDED_SEARCH) return -1;
+    int slot = -1;
+    for (uint32_t s = 0; s < P.nslots; ++s)
+        if (P.slot_a[s] == a && P.slot_b[s] == b) slot = (int)s;
+    if (slot < 0) return -1;
+    const uint32_t rows = P.dom_size[a];
+    const uint32_t total = ctx->last_state[0].aux_total[slot];
+    int rc = guarded(ctx, [&]() -> int {
+        if (off) HIPX(hipMemcpy(off, P.aux_off[slot], std::min(rows + 1, off_cap) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (idx && total)
+            HIPX(hipMemcpy(idx, ctx->d_pool + ctx->last_state[0].aux_base[slot], std::min(total, idx_cap) * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost));
+        return TFSM_OK;
+    });
+    return rc == TFSM_OK ? (int64_t)total : rc;
+}
+
+int tfsm_last_order(tfsm_ctx *ctx, uint32_t u, uint32_t *order) {
+    if (!ctx || !order || ctx->last_static.size() != 1 || u >= ctx->last_static[0].nv) return TFSM_ERR_ARGUMENT;
+    const PatStatic &P = ctx->last_static[0];
+    for (uint32_t d = 0; d < P.nv; ++d) order[d] = P.order[u][d];
+    return TFSM_OK;
+}
+
+int64_t tfsm_read_trace(tfsm_ctx *ctx, uint32_t *out, uint64_t cap) {
+    if (!ctx) return TFSM_ERR_ARGUMENT;
+    const uint64_t n = ctx->trace_len;
+    int rc = guarded(ctx, [&]() -> int {
+        if (out && n) HIPX(hipMemcpy(out, ctx->d_trace, std::min(n, cap) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        return TFSM_OK;
+    });
+    return rc == TFSM_OK ? (int64_t)n : rc;
+}
+
+int tfsm_last_stats(tfsm_ctx *ctx, tfsm_call_stats *out) {
+    if (!ctx || !out) return TFSM_ERR_ARGUMENT;
+    *out = ctx->stats;
+    return TFSM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+// libtfsm_host.so: C entry points over the host side (loader, pattern growth,
+// miner) for Python callers (tests, bench).  The GPU evaluation itself is
+// reached only through libtfsm_gpu.so's C ABI.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "graph.h"
+#include "miner.h"
+#include "pattern.h"
+
+extern "C" {
+
+struct tfsmh_graph {
+    tfsm::DataGraph g;
+    std::string err;
+};
+
+static std::string g_err;
+
+const char *tfsmh_last_error() { return g_err.c_str(); }
+
+tfsmh_graph *tfsmh_load(const char *path, int freq) {
+    try {
+        auto *h = new tfsmh_graph;
+        h->g = tfsm::prune_graph(tfsm::read_lg(path), freq);
+        return h;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void tfsmh_free(tfsmh_graph *h) { delete h; }
+
+// n, num_adj, num_vlabels, num_elabels, num_edge_stats
+void tfsmh_info(const tfsmh_graph *h, uint64_t *out5) {
+    out5[0] = h->g.n;
+    out5[1] = h->g.nbrs.size();
+    out5[2] = h->g.vlabel_values.size();
+    out5[3] = h->g.elabel_values.size();
+    out5[4] = h->g.edge_stats.size();
+}
+
+// Dense CSR plus label value tables; arrays sized from tfsmh_info.
+void tfsmh_csr(const tfsmh_graph *h, uint32_t *offsets, uint32_t *nbrs, uint32_t *elabels, uint32_t *vlabels,
+               int32_t *vlabel_values, int32_t *elabel_values) {
+    const auto &g = h->g;
+    std::memcpy(offsets, g.offsets.data(), g.offsets.size() * 4);
+    if (!g.nbrs.empty()) {
+        std::memcpy(nbrs, g.nbrs.data(), g.nbrs.size() * 4);
+        std::memcpy(elabels, g.elab.data(), g.elab.size() * 4);
+    }
+    if (g.n) std::memcpy(vlabels, g.vlabel.data(), g.n * 4);
+    std::memcpy(vlabel_values, g.vlabel_values.data(), g.vlabel_values.size() * 4);
+    std::memcpy(elabel_values, g.elabel_values.data(), g.elabel_values.size() * 4);
+}
+
+// 5 int32 per entry: la, el, lb, |A|, |B|
+void tfsmh_edge_stats(const tfsmh_graph *h, int32_t *out) {
+    size_t k = 0;
+    for (auto &e : h->g.edge_stats) {
+        out[k++] = e.la; out[k++] = e.el; out[k++] = e.lb;
+        out[k++] = (int32_t)e.na; out[k++] = (int32_t)e.nb;
+    }
+}
+
+int tfsmh_ctx_create(const tfsmh_graph *h, tfsm_ctx **out) {
+    tfsm_csr c = tfsm::csr_view(h->g);
+    return tfsm_ctx_create(&c, out);
+}
+
+// Runs the miner.  stats_out (15 doubles): mining_s, evaluated, frequent,
+// verdicts, searches, eval_calls, filter_ms, table_ms, search_ms,
+// search_launches, bytes_table, bytes_search, levels, max pattern size, 0.
+// results_path: grami_results.txt format; keys_path: one line per frequent
+// pattern "<dfs key>\t<support lower>\t<support upper>" (either may be NULL).
+int64_t tfsmh_mine(const tfsmh_graph *h, tfsm_ctx *ctx, int freq, int max_nodes, int threads, int batched, int mode,
+                   int learned, const char *results_path, const char *keys_path, double *stats_out) {
+    tfsm::MinerOptions opt;
+    opt.freq = freq;
+    opt.max_nodes = max_nodes;
+    opt.threads = threads;
+    opt.batched = batched != 0;
+    opt.mode = mode;
+    opt.learned = learned != 0;
+    std::vector<tfsm::FoundPattern> found;
+    tfsm::MinerStats st;
+    std::string err;
+    int rc = tfsm::mine(h->g, ctx, opt, found, st, err);
+    if (rc != 0) {
+        g_err = err;
+        return rc;
+    }
+    if (results_path) tfsm::write_results(found, results_path);
+    if (keys_path) {
+        FILE *f = std::fopen(keys_path, "w");
+        if (f) {
+            for (auto &fp : found) std::fprintf(f, "%s\t%u\t%u\n", fp.p.key().c_str(), fp.support_lower, fp.support_upper);
+            std::fclose(f);
+        }
+    }
+    if (stats_out) {
+        double v[15] = {st.mining_s, (double)st.evaluated, (double)st.frequent, (double)st.verdicts, (double)st.searches,
+                        (double)st.eval_calls, st.gpu_ms[0], st.gpu_ms[1], st.gpu_ms[2], (double)st.search_launches,
+                        (double)st.bytes_table, (double)st.bytes_search, (double)st.levels.size(),
+                        st.levels.empty() ? 0.0 : (double)st.levels.back().size, 0.0};
+        std::memcpy(stats_out, v, sizeof(v));
+    }
+    return (int64_t)found.size();
+}
+
+// Minimum-DFS-code test of a code given as 5 ints per edge (from, to, fl, el, tl).
+int tfsmh_is_min_code(const int32_t *code, int ne) {
+    try {
+        tfsm::Pattern p;
+        for (int i = 0; i < ne; ++i)
+            p.push(tfsm::DfsCode{code[5 * i], code[5 * i + 1], code[5 * i + 2], code[5 * i + 3], code[5 * i + 4]});
+        return tfsm::is_min_code(p) ? 1 : 0;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+// Level-synchronous miner; behaviour cited in miner.h.
+#include "miner.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <thread>
+
+namespace tfsm {
+
+tfsm_csr csr_view(const DataGraph &g) {
+    tfsm_csr c;
+    c.num_vertices = g.n;
+    c.num_adj = g.nbrs.size();
+    c.offsets = g.offsets.data();
+    c.nbrs = g.nbrs.data();
+    c.elabels = g.elab.data();
+    c.vlabels = g.vlabel.data();
+    c.num_vlabels = (uint32_t)g.vlabel_values.size();
+    c.num_elabels = (uint32_t)g.elabel_values.size();
+    return c;
+}
+
+namespace {
+
+struct Candidate {
+    Pattern p;
+    uint64_t parent_handle = 0;
+};
+
+// Dense-label form of a pattern; false when a label is absent from the pruned
+// graph (then the pattern has no embedding and is infrequent).
+bool to_device_pattern(const DataGraph &g, const Pattern &p, tfsm_pattern &out) {
+    std::memset(&out, 0, sizeof(out));
+    out.num_vertices = (uint32_t)p.nv();
+    out.num_edges = (uint32_t)p.ne();
+    for (int i = 0; i < p.nv(); ++i) {
+        int d = g.dense_vlabel(p.vlab[i]);
+        if (d < 0) return false;
+        out.vlabel[i] = (uint32_t)d;
+    }
+    for (int e = 0; e < p.ne(); ++e) {
+        int d = g.dense_elabel(p.code[e].el);
+        if (d < 0) return false;
+        out.edge_a[e] = (uint32_t)p.code[e].from;
+        out.edge_b[e] = (uint32_t)p.code[e].to;
+        out.edge_label[e] = (uint32_t)d;
+    }
+    return true;
+}
+
+template <class F>
+void parallel_for(size_t n, int threads, F &&f) {
+    if (threads <= 1 || n < 2) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&]() {
+            size_t i;
+            while ((i = next.fetch_add(1)) < n) f(i);
+        });
+    for (auto &t : pool) t.join();
+}
+
+}  // namespace
+
+int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector<FoundPattern> &found,
+         MinerStats &stats, std::string &err) {
+    auto t0 = std::chrono::steady_clock::now();
+    stats = MinerStats{};
+    found.clear();
+    GrowthContext gc;
+    gc.max_nodes = opt.max_nodes;
+    for (int32_t l : g.elabel_values) gc.freq_edge_labels.insert(l);
+    // size-2 level (grami.h:83-169): every surviving endpoint-set entry is frequent
+    std::vector<Pattern> size2;
+    for (const EdgeStat &e : g.edge_stats) {
+        gc.neighbor_labels[e.la].insert(e.lb);
+        gc.neighbor_labels[e.lb].insert(e.la);
+        FoundPattern f;
+        f.p = single_edge(e.la, e.el, e.lb);
+        f.support_lower = f.support_upper = std::min(e.na, e.nb);
+        size2.push_back(f.p);
+        found.push_back(std::move(f));
+    }
+    stats.frequent = found.size();
+    stats.levels.push_back(LevelStats{2, 0, found.size(), 0});
+
+    std::vector<Candidate> level;
+    {
+        std::vector<std::vector<Pattern>> kids(size2.size());
+        parallel_for(size2.size(), opt.threads, [&](size_t i) { kids[i] = extend(size2[i], gc); });
+        for (auto &k : kids)
+            for (auto &c : k) level.push_back(Candidate{std::move(c), 0});
+    }
+    tfsm_eval_options eo;
+    std::memset(&eo, 0, sizeof(eo));
+    eo.mode = (uint32_t)opt.mode;
+    eo.retain_domains = 1;
+    eo.task_order = opt.learned ? TFSM_ORDER_LEARNED : TFSM_ORDER_REFERENCE;
+    eo.trace = opt.trace ? 1 : 0;
+
+    std::vector<uint64_t> live_handles;
+    while (!level.empty()) {
+        auto tl = std::chrono::steady_clock::now();
+        const size_t n = level.size();
+        std::vector<tfsm_pattern> dp(n);
+        std::vector<uint64_t> parents(n);
+        std::vector<char> ok(n);
+        for (size_t i = 0; i < n; ++i) {
+            ok[i] = to_device_pattern(g, level[i].p, dp[i]);
+            parents[i] = level[i].parent_handle;
+        }
+        std::vector<tfsm_pattern_result> res(n);
+        std::memset(res.data(), 0, n * sizeof(tfsm_pattern_result));
+        auto account = [&]() {
+            tfsm_call_stats cs;
+            tfsm_last_stats(ctx, &cs);
+            stats.eval_calls++;
+            stats.gpu_ms[0] += cs.filter_ms;
+            stats.gpu_ms[1] += cs.table_ms;
+            stats.gpu_ms[2] += cs.search_ms;
+            stats.search_launches += cs.search_launches;
+            stats.search_kernel_ms += cs.search_ms;
+            stats.bytes_table += cs.table_bytes;
+            stats.bytes_search += cs.search_bytes;
+        };
+        if (opt.batched) {
+            std::vector<tfsm_pattern> sub;
+            std::vector<uint64_t> subpar;
+            std::vector<size_t> where;
+            for (size_t i = 0; i < n; ++i)
+                if (ok[i]) { sub.push_back(dp[i]); subpar.push_back(parents[i]); where.push_back(i); }
+            std::vector<tfsm_pattern_result> subres(sub.size());
+            if (!sub.empty()) {
+                int rc = tfsm_eval_level(ctx, sub.data(), subpar.data(), (uint32_t)sub.size(), (uint32_t)opt.freq, &eo,
+                                         subres.data());
+                if (rc != TFSM_OK) { err = tfsm_last_error(ctx); return rc; }
+                account();
+            }
+            for (size_t k = 0; k < where.size(); ++k) res[where[k]] = subres[k];
+        } else {
+            for (size_t i = 0; i < n; ++i) {
+                if (!ok[i]) continue;
+                int rc = tfsm_eval_pattern(ctx, &dp[i], parents[i], (uint32_t)opt.freq, &eo, &res[i]);
+                if (rc != TFSM_OK) { err = tfsm_last_error(ctx); return rc; }
+                account();
+            }
+        }
+        // the parents' domains are no longer needed once all their children ran
+        for (uint64_t h : live_handles) tfsm_domain_release(ctx, h);
+        live_handles.clear();
+
+        LevelStats ls;
+        ls.size = level.empty() ? 0 : (uint32_t)level[0].p.nv();
+        ls.evaluated = n;
+        std::vector<size_t> freq_idx;
+        for (size_t i = 0; i < n; ++i) {
+            stats.evaluated++;
+            stats.verdicts += res[i].verdicts;
+            stats.searches += res[i].searches;
+            if (!res[i].frequent) continue;
+            freq_idx.push_back(i);
+            if (res[i].domain_handle) live_handles.push_back(res[i].domain_handle);
+        }
+        ls.frequent = freq_idx.size();
+        std::vector<std::vector<Pattern>> kids(freq_idx.size());
+        parallel_for(freq_idx.size(), opt.threads, [&](size_t k) { kids[k] = extend(level[freq_idx[k]].p, gc); });
+        std::vector<Candidate> next;
+        for (size_t k = 0; k < freq_idx.size(); ++k) {
+            const size_t i = freq_idx[k];
+            FoundPattern f;
+            f.p = level[i].p;
+            f.support_lower = res[i].support_lower;
+            f.support_upper = res[i].support_upper;
+            found.push_back(std::move(f));
+            for (auto &c : kids[k]) next.push_back(Candidate{std::move(c), res[i].domain_handle});
+        }
+        stats.frequent += freq_idx.size();
+        ls.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tl).count();
+        stats.levels.push_back(ls);
+        level.swap(next);
+    }
+    for (uint64_t h : live_handles) tfsm_domain_release(ctx, h);
+    std::stable_sort(found.begin(), found.end(), [](const FoundPattern &a, const FoundPattern &b) {
+        if (a.p.nv() != b.p.nv()) return a.p.nv() < b.p.nv();
+        return a.p.key() < b.p.key();
+    });
+    stats.mining_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+void write_results(const std::vector<FoundPattern> &found, const std::string &path) {
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) return;
+    for (auto &fp : found) {
+        std::fputs("=====================================\n", f);
+        std::fputs(fp.p.to_lg_block().c_str(), f);
+    }
+    std::fclose(f);
+}
+
+}  // namespace tfsm

@@ -1,0 +1,70 @@
+// Level-synchronous MNI miner on top of the GPU evaluation library.
+//
+// Replaces the reference's Worker / Comper task loop (worker.h:4-166,
+// comper.h:1302-1715) for this build: the size-2 level comes from the edge
+// statistics (grami.h:83-169, 459-481), every later level is the set of
+// canonical rightmost extensions of the previous level's frequent patterns
+// (grami.h:172-294), evaluated on the GPU either one pattern per launch
+// sequence (plain) or the whole level in one fused sequence (batched).  A
+// child's first candidate lists start from its parent's domains minus the
+// parent's proven-invalid candidates (comper.h:1069-1152); children of
+// size-2 patterns start from the label buckets (worker.h:57-106).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "graph.h"
+#include "pattern.h"
+#include "../../include/tfsm_gpu.h"
+
+namespace tfsm {
+
+struct MinerOptions {
+    int32_t freq = 0;
+    int max_nodes = -1;
+    int threads = 1;           // host threads for pattern growth
+    bool batched = false;      // one eval call per level instead of per pattern
+    int mode = TFSM_MODE_CAPPED;
+    bool learned = false;      // learned task order (requires a ranker on the context)
+    bool trace = false;        // keep per-task traces for ranker training
+};
+
+struct FoundPattern {
+    Pattern p;
+    uint32_t support_lower = 0, support_upper = 0;
+};
+
+struct LevelStats {
+    uint32_t size = 0;          // pattern vertex count of the level
+    uint64_t evaluated = 0, frequent = 0;
+    double wall_ms = 0;
+};
+
+struct MinerStats {
+    double mining_s = 0;
+    uint64_t evaluated = 0;         // candidate patterns evaluated (size >= 3)
+    uint64_t frequent = 0;          // frequent patterns, size-2 included
+    uint64_t verdicts = 0;          // final (u, v) verdicts
+    uint64_t searches = 0;          // existence searches
+    uint64_t eval_calls = 0;        // tfsm_eval_* calls
+    double gpu_ms[3] = {0, 0, 0};   // filter, table, search (device time)
+    uint64_t search_launches = 0;
+    double search_kernel_ms = 0;    // sum of search-kernel launch durations
+    uint64_t bytes_table = 0, bytes_search = 0;   // algorithmic bytes
+    std::vector<LevelStats> levels;
+};
+
+// Runs the whole mining job; `found` receives the frequent patterns in a
+// deterministic order (by size, then DFS-code key).  Returns 0 or a tfsm_status.
+int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector<FoundPattern> &found,
+         MinerStats &stats, std::string &err);
+
+// Host CSR view of the pruned graph for tfsm_ctx_create.
+tfsm_csr csr_view(const DataGraph &g);
+
+// grami_results.txt in the reference serial build's format (serial/grami.h:622-635).
+void write_results(const std::vector<FoundPattern> &found, const std::string &path);
+
+}  // namespace tfsm

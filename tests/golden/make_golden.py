@@ -1,0 +1,133 @@
+"""Writes the golden pattern-set fixtures from the reference T-FSM binaries.
+
+Runs oracle/_ref/run_serial and oracle/_ref/run_parallel (built from the
+reference sources by `make -f oracle/ref.mk`) on the seeded workloads written
+by t-fsm_amd/_build/tfsm_gen, and stores each frequent-pattern set as sorted
+DFS-code keys "from,to,from_label,edge_label,to_label;..." (one pattern per
+line) in tests/golden/<name>.keys, with tests/golden/manifest.json recording
+the command, the reference's own reported count and a sha256 of the keys.
+
+    python tests/golden/make_golden.py [name ...]
+
+The serial build writes grami_results.txt (serial/grami.h:622-635) with every
+frequent pattern.  The parallel build prints patterns of three or more
+vertices between `~~~~~` / `#####` markers (parallel/comper.h:1027-1031,
+1095-1101) and only counts the size-2 patterns, so its fixtures hold the
+patterns of three or more vertices plus the reported total.
+"""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+GEN = os.path.join(REPO, "t-fsm_amd", "_build", "tfsm_gen")
+REF = os.path.join(REPO, "oracle", "_ref")
+
+# name: (workload, seed, freq, maxNodes, reference build, threads)
+CONFIGS = {
+    "readme_f1_m3": ("readme", 0, 1, 3, "serial", 1),
+    "readme_f2_m4": ("readme", 0, 2, 4, "serial", 1),
+    "gnm2k_f150_m4": ("gnm2k", 1, 150, 4, "serial", 1),
+    "gnm2k_f80_m4": ("gnm2k", 1, 80, 4, "serial", 1),
+    "gnm2k_f40_m4": ("gnm2k", 1, 40, 4, "serial", 1),
+    "powerlaw100k_f9000_m5": ("powerlaw100k", 2, 9000, 5, "serial", 1),
+    "powerlaw100k_f6000_m6": ("powerlaw100k", 2, 6000, 6, "parallel", 8),
+}
+
+
+def key_of(vl, es):
+    return "".join(f"{a},{b},{vl[a]},{el},{vl[b]};" for a, b, el in es)
+
+
+def parse_serial(path):
+    keys, vl, es = [], {}, []
+    for line in open(path):
+        t = line.split()
+        if not t:
+            continue
+        if t[0].startswith("====="):
+            if es:
+                keys.append(key_of(vl, es))
+            vl, es = {}, []
+        elif t[0] == "v":
+            vl[int(t[1])] = int(t[2])
+        elif t[0] == "e":
+            es.append((int(t[1]), int(t[2]), int(t[3])))
+    if es:
+        keys.append(key_of(vl, es))
+    return keys
+
+
+def parse_parallel(text):
+    keys, cur = [], None
+    for line in text.split("\n"):
+        if line in ("~~~~~", "#####"):
+            if cur is None:
+                cur = []
+            else:
+                vl, es = {}, []
+                for l in cur:
+                    t = l.split()
+                    if len(t) == 3 and t[0] == "v":
+                        vl[int(t[1])] = int(t[2])
+                    elif len(t) == 4 and t[0] == "e":
+                        es.append((int(t[1]), int(t[2]), int(t[3])))
+                    else:
+                        raise ValueError(f"interleaved output line {l!r}")
+                keys.append(key_of(vl, es))
+                cur = None
+        elif cur is not None:
+            cur.append(line)
+    return keys
+
+
+def graph_file(workload, seed, tmp):
+    if workload == "readme":
+        return os.path.join(HERE, "readme_sample.lg")
+    path = os.path.join(tmp, f"{workload}_{seed}.lg")
+    if not os.path.exists(path):
+        subprocess.run([GEN, workload, path, str(seed)], check=True, stdout=subprocess.DEVNULL)
+    return path
+
+
+def main(names):
+    manifest_path = os.path.join(HERE, "manifest.json")
+    manifest = json.load(open(manifest_path)) if os.path.exists(manifest_path) else {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name in names or CONFIGS:
+            workload, seed, freq, maxn, build, threads = CONFIGS[name]
+            lg = graph_file(workload, seed, tmp)
+            t0 = time.time()
+            if build == "serial":
+                cmd = [os.path.join(REF, "run_serial"), "-file", lg, "-freq", str(freq), "-maxNodes", str(maxn)]
+                out = subprocess.run(cmd, cwd=tmp, check=True, capture_output=True, text=True).stdout
+                keys = parse_serial(os.path.join(tmp, "grami_results.txt"))
+                reported = int([l for l in out.split("\n") if "# Frequent Patterns" in l][0].split(":")[1])
+            else:
+                cmd = [os.path.join(REF, "run_parallel"), "-file", lg, "-freq", str(freq), "-thread", str(threads),
+                       "-maxNodes", str(maxn)]
+                out = subprocess.run(cmd, cwd=tmp, check=True, capture_output=True, text=True).stdout
+                keys = parse_parallel(out)
+                reported = int([l for l in out.split("\n") if "[INFO] # Frequent Patterns" in l][0].split(":")[1])
+            keys = sorted(set(keys))
+            body = "".join(k + "\n" for k in keys)
+            with open(os.path.join(HERE, name + ".keys"), "w") as f:
+                f.write(body)
+            manifest[name] = dict(workload=workload, seed=seed, freq=freq, max_nodes=maxn, reference_build=build,
+                                  threads=threads, reported_total=reported, keys=len(keys),
+                                  keys_hold="all frequent patterns" if build == "serial" else
+                                  "frequent patterns with three or more vertices",
+                                  sha256=hashlib.sha256(body.encode()).hexdigest(),
+                                  seconds=round(time.time() - t0, 2))
+            print(name, manifest[name])
+    with open(manifest_path, "w") as f:
+        json.dump(manifest, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -1,0 +1,62 @@
+"""Whole mining runs on the GPU against the reference's golden pattern sets and
+the oracle's exact supports.
+
+Pattern sets must equal the reference binaries' sets exactly (plain and
+batched paths, capped and exact support modes).  In exact mode every frequent
+pattern's MNI support must equal the oracle's exact-mode support.
+"""
+import os
+
+import pytest
+
+import oracle as orc
+import tfsm
+from conftest import golden_keys, golden_manifest, workload_path
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["readme_f1_m3", "readme_f2_m4", "gnm2k_f150_m4", "gnm2k_f80_m4", "gnm2k_f40_m4",
+         "powerlaw100k_f9000_m5"]
+
+
+def read_keys(path):
+    out = {}
+    for line in open(path):
+        k, lo, hi = line.rstrip("\n").split("\t")
+        out[k] = (int(lo), int(hi))
+    return out
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("batched", [False, True])
+def test_pattern_sets(name, batched, workdir):
+    m = golden_manifest()[name]
+    path = workload_path(m["workload"], m["seed"], workdir)
+    g = tfsm.Graph(path, m["freq"])
+    ctx = tfsm.Context(g)
+    keys_path = os.path.join(str(workdir), f"{name}_{int(batched)}.keys")
+    n, stats = ctx.mine(m["freq"], m["max_nodes"], threads=4, batched=batched, mode=tfsm.MODE_CAPPED,
+                        keys_path=keys_path)
+    mine = set(read_keys(keys_path))
+    assert n == m["reported_total"]
+    assert mine == golden_keys(name)
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["gnm2k_f80_m4", "powerlaw100k_f9000_m5"])
+def test_exact_supports(name, workdir):
+    m = golden_manifest()[name]
+    path = workload_path(m["workload"], m["seed"], workdir)
+    g = tfsm.Graph(path, m["freq"])
+    ctx = tfsm.Context(g)
+    kp = os.path.join(str(workdir), f"{name}_exact.keys")
+    ctx.mine(m["freq"], m["max_nodes"], threads=4, batched=True, mode=tfsm.MODE_EXACT, keys_path=kp)
+    og = orc.OracleGraph(path, m["freq"])
+    op = os.path.join(str(workdir), f"{name}_oracle_exact.keys")
+    og.mine(m["freq"], m["max_nodes"], 8, 1, op)
+    mine, theirs = read_keys(kp), read_keys(op)
+    assert set(mine) == set(theirs) == golden_keys(name)
+    for k in mine:
+        assert mine[k][0] == mine[k][1], k          # exact mode: lower == upper == MNI
+        assert mine[k] == theirs[k], k
+    ctx.close()
