@@ -89,7 +89,9 @@ typedef struct {
     uint32_t trace;             /* record per-task verdict and step count (tfsm_read_trace) */
     uint32_t block_waves;       /* waves per workgroup of the search kernel (0: tuned default) */
     uint32_t grid_waves_per_cu; /* resident waves per CU of the search kernel (0: tuned default) */
-    uint32_t split_steps;       /* reserved (0) */
+    uint32_t split_steps;       /* DFS steps after which a search donates its untried
+                                   candidates to idle lanes (0: default 4096,
+                                   0xffffffff: never); verdicts do not depend on it */
     uint32_t reserved;
 } tfsm_eval_options;
 
@@ -166,6 +168,8 @@ typedef struct {
     uint32_t search_launches;
     uint64_t table_bytes;
     uint64_t search_bytes;
+    uint32_t search_splits;     /* sub-tree pieces donated between lanes */
+    uint32_t search_watchdog;   /* 1 when a drained wave left on its idle limit */
 } tfsm_call_stats;
 int tfsm_last_stats(tfsm_ctx *ctx, tfsm_call_stats *out);
 

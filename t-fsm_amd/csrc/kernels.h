@@ -96,16 +96,40 @@ struct WorkTable {
     unsigned long long total;
 };
 
+// A donated piece of one (u, v) search: the candidates [lo, hi) at depth
+// `floor` of the matching order of u, below the fixed prefix of candidate
+// indices chosen at depths 0 .. floor-1.
+struct SplitRecord {
+    uint32_t seq;                          // 1 once the record is written
+    uint32_t p, u, ci, floor, cnt_id, lo, hi;
+    uint32_t prefix[kMaxV];
+};
+
+// Global bookkeeping of one search launch (zeroed before each launch).
+struct SearchCounters {
+    unsigned long long main_next;          // next root task
+    unsigned long long pending;            // running pieces + reserved records
+    uint32_t split_tail;                   // records reserved by producers
+    uint32_t split_head;                   // tickets taken by consumers
+    uint32_t cnt_alloc;                    // piece counters handed out
+    uint32_t watchdog;                     // set when a wave left on its idle limit
+};
+
 struct VerifyArgs {
     const PatStatic *ps;
     PatState *st;
     const unsigned long long *task_base;   // P + 1
     uint32_t P;
     unsigned long long total;
-    unsigned long long *counter;
+    SearchCounters *ctr;
     const uint32_t *pool;
     uint32_t *trace;                       // nullable, indexed by reference task order
     uint32_t steps_per_round;
+    uint32_t split_steps;                  // donate work after this many steps (0: never)
+    SplitRecord *records;                  // split_cap records
+    uint32_t split_cap;
+    uint32_t *piece_cnt;                   // split_cap counters
+    uint32_t *piece_incomplete;            // split_cap flags
 };
 
 }  // namespace tfsm_dev

@@ -21,6 +21,11 @@
 
 using namespace tfsm_dev;
 
+// Search-kernel work splitting: record capacity per launch and the default
+// step budget after which a piece donates work.
+constexpr uint32_t kSplitCap = 1u << 21;
+constexpr uint32_t kDefaultSplitSteps = 4096;
+
 namespace {
 
 struct HipError {
@@ -142,9 +147,13 @@ struct tfsm_ctx {
     uint32_t *d_pool = nullptr;
     uint32_t pool_cap = 0;
     uint32_t *d_flags = nullptr;             // [0] pool_used, [1] overflow, [2..] prune changed flags
-    unsigned long long *d_counter = nullptr;
+    SearchCounters *d_ctr = nullptr;
+    SplitRecord *d_records = nullptr;        // kSplitCap records, reused across launches by epoch
+    uint32_t *d_piece_cnt = nullptr, *d_piece_inc = nullptr;
+    uint32_t epoch = 0;
     Pinned<uint32_t> h_flags;
     Pinned<PatState> h_state;
+    Pinned<SearchCounters> h_ctr;
     // retained domains
     BufferCache cache;
     std::unordered_map<uint64_t, Retained> retained;
@@ -537,24 +546,34 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         va.task_base = upload(c, task_base);
         va.P = n;
         va.total = total_tasks;
-        va.counter = c->d_counter;
+        va.ctr = c->d_ctr;
         va.pool = c->d_pool;
         va.trace = opt.trace ? c->d_trace : nullptr;
         va.steps_per_round = 4;
-        HIPX(hipMemsetAsync(c->d_counter, 0, sizeof(unsigned long long), s));
+        va.split_steps = opt.split_steps == 0 ? kDefaultSplitSteps : opt.split_steps;
+        if (opt.split_steps == 0xffffffffu) va.split_steps = 0;
+        va.records = c->d_records;
+        va.split_cap = kSplitCap;
+        va.piece_cnt = c->d_piece_cnt;
+        va.piece_incomplete = c->d_piece_inc;
+        const uint32_t epoch = ++c->epoch;
+        HIPX(hipMemsetAsync(c->d_ctr, 0, sizeof(SearchCounters), s));
         HIPX(hipEventRecord(c->ev[4], s));
         uint32_t max_nv = 0;
         for (uint32_t p = 0; p < n; ++p) if (decided[p] == TFSM_DECIDED_SEARCH) max_nv = std::max(max_nv, hs[p].nv);
-        const uint32_t per_cu = opt.grid_waves_per_cu ? std::max(1u, opt.grid_waves_per_cu / (kBlock / 64)) : 4u;
-        unsigned long long want = (total_tasks + kBlock - 1) / kBlock;
-        uint32_t grid = (uint32_t)std::max<unsigned long long>(1, std::min<unsigned long long>(want, (unsigned long long)c->num_cus * per_cu));
-        if (max_nv <= 8) hipLaunchKernelGGL(k_verify<8>, dim3(grid), dim3(kBlock), 0, s, va);
-        else hipLaunchKernelGGL(k_verify<16>, dim3(grid), dim3(kBlock), 0, s, va);
+        // persistent grid: resident blocks per CU (LDS-bound for the 16-deep variant)
+        const uint32_t per_cu = opt.grid_waves_per_cu ? std::max(1u, opt.grid_waves_per_cu / (kBlock / 64))
+                                                      : (max_nv <= 8 ? 4u : 2u);
+        const uint32_t grid = (uint32_t)c->num_cus * per_cu;
+        if (max_nv <= 8) hipLaunchKernelGGL(k_verify<8>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
+        else hipLaunchKernelGGL(k_verify<16>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
         HIPX(hipGetLastError());
         HIPX(hipEventRecord(c->ev[5], s));
     }
     HIPX(hipEventRecord(c->ev[3], s));
     HIPX(hipMemcpyAsync(hst, d_st, n * sizeof(PatState), hipMemcpyDeviceToHost, s));
+    SearchCounters *hctr = c->h_ctr.ensure(1);
+    if (total_tasks) HIPX(hipMemcpyAsync(hctr, c->d_ctr, sizeof(SearchCounters), hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
     std::memset(&c->stats, 0, sizeof(c->stats));
     HIPX(hipEventElapsedTime(&c->stats.filter_ms, c->ev[0], c->ev[1]));
@@ -562,6 +581,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
     if (total_tasks) {
         HIPX(hipEventElapsedTime(&c->stats.search_ms, c->ev[4], c->ev[5]));
         c->stats.search_launches = 1;
+        c->stats.search_splits = std::min(hctr->split_tail, kSplitCap);
+        c->stats.search_watchdog = hctr->watchdog ? 1u : 0u;
     }
     {
         // algorithmic bytes (definitions in tfsm_gpu.h)
@@ -744,7 +765,12 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         c->pool_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(4 * g->num_adj, 1u << 22), 0xfffffff0ull);
         HIPX(hipMalloc(&c->d_pool, (size_t)c->pool_cap * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_flags, 16 * sizeof(uint32_t)));
-        HIPX(hipMalloc(&c->d_counter, sizeof(unsigned long long)));
+        HIPX(hipMalloc(&c->d_ctr, sizeof(SearchCounters)));
+        HIPX(hipMalloc(&c->d_records, (size_t)kSplitCap * sizeof(SplitRecord)));
+        HIPX(hipMalloc(&c->d_piece_cnt, (size_t)kSplitCap * sizeof(uint32_t)));
+        HIPX(hipMalloc(&c->d_piece_inc, (size_t)kSplitCap * sizeof(uint32_t)));
+        HIPX(hipMemset(c->d_records, 0, (size_t)kSplitCap * sizeof(SplitRecord)));
+        HIPX(hipMemset(c->d_piece_inc, 0, (size_t)kSplitCap * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_scanned, sizeof(unsigned long long)));
         return TFSM_OK;
     });
@@ -766,7 +792,10 @@ int tfsm_ctx_destroy(tfsm_ctx *c) {
     (void)hipFree(c->d_bucket);
     (void)hipFree(c->d_pool);
     (void)hipFree(c->d_flags);
-    (void)hipFree(c->d_counter);
+    (void)hipFree(c->d_ctr);
+    (void)hipFree(c->d_records);
+    (void)hipFree(c->d_piece_cnt);
+    (void)hipFree(c->d_piece_inc);
     (void)hipFree(c->d_scanned);
     (void)hipFree(c->d_trace);
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);

@@ -110,6 +110,37 @@ int64_t tfsmh_mine(const tfsmh_graph *h, tfsm_ctx *ctx, int freq, int max_nodes,
     return (int64_t)found.size();
 }
 
+// Canonical rightmost extensions of one pattern (codes as 5 ints per edge),
+// with the label pruning learnt from the graph's frequent size-2 patterns.
+// Writes children back to back (each ne + 1 codes) into out (cap ints);
+// returns the number of children or -1 when out is too small.
+int tfsmh_extend(const tfsmh_graph *h, int max_nodes, const int32_t *code, int ne, int32_t *out, int cap) {
+    try {
+        tfsm::GrowthContext gc;
+        gc.max_nodes = max_nodes;
+        for (int32_t l : h->g.elabel_values) gc.freq_edge_labels.insert(l);
+        for (auto &e : h->g.edge_stats) {
+            gc.neighbor_labels[e.la].insert(e.lb);
+            gc.neighbor_labels[e.lb].insert(e.la);
+        }
+        tfsm::Pattern p;
+        for (int i = 0; i < ne; ++i)
+            p.push(tfsm::DfsCode{code[5 * i], code[5 * i + 1], code[5 * i + 2], code[5 * i + 3], code[5 * i + 4]});
+        std::vector<tfsm::Pattern> kids = tfsm::extend(p, gc);
+        int k = 0;
+        for (auto &c : kids) {
+            if (k + 5 * c.ne() > cap) return -1;
+            for (auto &d : c.code) {
+                out[k++] = d.from; out[k++] = d.to; out[k++] = d.fl; out[k++] = d.el; out[k++] = d.tl;
+            }
+        }
+        return (int)kids.size();
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 // Minimum-DFS-code test of a code given as 5 ints per edge (from, to, fl, el, tl).
 int tfsmh_is_min_code(const int32_t *code, int ne) {
     try {

@@ -59,7 +59,8 @@ class PatternResult(ctypes.Structure):
 class CallStats(ctypes.Structure):
     _fields_ = [("filter_ms", ctypes.c_float), ("table_ms", ctypes.c_float), ("search_ms", ctypes.c_float),
                 ("search_launches", ctypes.c_uint32), ("table_bytes", ctypes.c_uint64),
-                ("search_bytes", ctypes.c_uint64)]
+                ("search_bytes", ctypes.c_uint64), ("search_splits", ctypes.c_uint32),
+                ("search_watchdog", ctypes.c_uint32)]
 
 
 # every symbol include/tfsm_gpu.h declares
@@ -132,6 +133,8 @@ def host_lib():
                                    ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, P(ctypes.c_double)]
         lib.tfsmh_mine.restype = ctypes.c_int64
         lib.tfsmh_is_min_code.argtypes = [P(ctypes.c_int32), ctypes.c_int]
+        lib.tfsmh_extend.argtypes = [vp, ctypes.c_int, P(ctypes.c_int32), ctypes.c_int, P(ctypes.c_int32),
+                                     ctypes.c_int]
         lib.tfsmh_last_error.restype = ctypes.c_char_p
         _host = lib
     return _host
@@ -177,6 +180,24 @@ class Graph:
         self.lib.tfsmh_edge_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
         return out[:5 * self.num_edge_stats].reshape(-1, 5)
 
+    def extend(self, codes, max_nodes=-1):
+        """Canonical children of a DFS code [(from, to, fl, el, tl), ...]."""
+        flat = (ctypes.c_int32 * (5 * len(codes)))(*[x for c in codes for x in c])
+        cap = 1 << 20
+        out = (ctypes.c_int32 * cap)()
+        n = self.lib.tfsmh_extend(self.h, int(max_nodes), flat, len(codes), out, cap)
+        if n < 0:
+            raise TfsmError(-4, "extension buffer too small")
+        kids, k, m = [], 0, 5 * (len(codes) + 1)
+        for _ in range(n):
+            kids.append([tuple(out[k + 5 * i:k + 5 * i + 5]) for i in range(len(codes) + 1)])
+            k += m
+        return kids
+
+    def size2_codes(self):
+        """DFS codes of the frequent single-edge patterns."""
+        return [[(0, 1, int(r[0]), int(r[1]), int(r[2]))] for r in self.edge_stats()]
+
     def close(self):
         if self.h:
             self.lib.tfsmh_free(self.h)
@@ -201,19 +222,22 @@ class Context:
             raise TfsmError(rc, "tfsm_ctx_create")
         self.h = h
 
-    def eval_pattern(self, pat, freq, mode=MODE_CAPPED, parent=0, retain=False, trace=False):
-        opt = EvalOptions(mode=mode, retain_domains=int(retain), task_order=ORDER_REFERENCE, trace=int(trace))
+    def eval_pattern(self, pat, freq, mode=MODE_CAPPED, parent=0, retain=False, trace=False, split_steps=0):
+        opt = EvalOptions(mode=mode, retain_domains=int(retain), task_order=ORDER_REFERENCE, trace=int(trace),
+                          split_steps=split_steps)
         res = PatternResult()
         _check(self.h, self.lib.tfsm_eval_pattern(self.h, ctypes.byref(pat), ctypes.c_uint64(parent), int(freq),
                                                   ctypes.byref(opt), ctypes.byref(res)))
         return res
 
-    def eval_level(self, pats, freq, mode=MODE_CAPPED, parents=None, retain=False, trace=False):
+    def eval_level(self, pats, freq, mode=MODE_CAPPED, parents=None, retain=False, trace=False, split_steps=0,
+                   grid_waves_per_cu=0):
         n = len(pats)
         arr = (Pattern * n)(*pats)
         par = (ctypes.c_uint64 * n)(*(parents or [0] * n))
         res = (PatternResult * n)()
-        opt = EvalOptions(mode=mode, retain_domains=int(retain), task_order=ORDER_REFERENCE, trace=int(trace))
+        opt = EvalOptions(mode=mode, retain_domains=int(retain), task_order=ORDER_REFERENCE, trace=int(trace),
+                          split_steps=split_steps, grid_waves_per_cu=grid_waves_per_cu)
         _check(self.h, self.lib.tfsm_eval_level(self.h, arr, par, n, int(freq), ctypes.byref(opt), res))
         return list(res)
 

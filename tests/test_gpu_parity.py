@@ -173,6 +173,25 @@ def test_level_equals_single(workdir):
                 assert list(b.valid_count) == list(s.valid_count)
 
 
+@pytest.mark.parametrize("split_steps", [64, 0xffffffff])
+def test_split_search_parity(split_steps, workdir):
+    """Sub-tree donation at a tiny step budget (every 64 steps) and no donation
+    at all must both give the oracle's exact per-candidate verdicts."""
+    env = env_for("powerlaw100k", 2, 9000, workdir)
+    for vl, es in variants(golden_keys("powerlaw100k_f9000_m5"), 15):
+        pat = env.dense(vl, es)
+        if pat is None:
+            continue
+        res = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_EXACT, split_steps=split_steps)
+        ref = env.og.eval(vl, es, env.freq, mode=1)
+        assert (res.support_lower, res.support_upper) == (ref.lower, ref.upper), f"{vl} {es}"
+        if ref.decided_by == 2:
+            for u in range(len(vl)):
+                assert np.array_equal(env.ctx.last_states(u), ref.states(u)), f"{vl} {es} verdicts {u}"
+        cap = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_CAPPED, split_steps=split_steps)
+        assert bool(cap.frequent) == ref.frequent
+
+
 def test_large_pattern_and_errors(workdir):
     """Nine-vertex paths use the 16-deep search kernel; malformed input is an error code."""
     env = env_for("gnm2k", 1, 40, workdir)
