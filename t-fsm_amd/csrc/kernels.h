@@ -70,6 +70,9 @@ struct PatStatic {
     uint8_t order[kMaxV][kMaxV];
     uint8_t bn_n[kMaxV][kMaxV];
     uint8_t bn_slot[kMaxV][kMaxV][kMaxV];
+    uint8_t sec_base[kMaxV][kMaxV];     // first cursor slot of depth d's non-primary rows
+    uint16_t same_before[kMaxV][kMaxV]; // earlier depths whose vertex has d's label
+    uint8_t same_after[kMaxV][kMaxV];   // 1 when a later depth has d's label
     uint32_t task_prefix[kMaxV + 1];    // reference task order: u-major prefix of |C(u)|
     const uint32_t *perm;               // optional task permutation (learned order)
 };
@@ -100,19 +103,28 @@ struct WorkTable {
 // `floor` of the matching order of u, below the fixed prefix of candidate
 // indices chosen at depths 0 .. floor-1.
 struct SplitRecord {
-    uint32_t seq;                          // 1 once the record is written
+    uint32_t seq;                          // the launch epoch once the record is written
     uint32_t p, u, ci, floor, cnt_id, lo, hi;
+    uint32_t prim;                         // which backward-neighbour row [lo, hi) indexes
     uint32_t prefix[kMaxV];
+    uint32_t pad[7];
 };
 
 // Global bookkeeping of one search launch (zeroed before each launch).
+// Each hot counter sits on its own 128-byte line so that polling one does not
+// slow the atomics on another.
 struct SearchCounters {
     unsigned long long main_next;          // next root task
-    unsigned long long pending;            // running pieces + reserved records
+    uint32_t pad0[30];
+    uint32_t busy_waves;                   // waves with running pieces or claimed records
+    uint32_t pad1[31];
     uint32_t split_tail;                   // records reserved by producers
-    uint32_t split_head;                   // tickets taken by consumers
+    uint32_t pad2[31];
+    uint32_t split_head;                   // records claimed by consumers (<= split_tail)
+    uint32_t pad3[31];
     uint32_t cnt_alloc;                    // piece counters handed out
     uint32_t watchdog;                     // set when a wave left on its idle limit
+    uint32_t pad4[30];
 };
 
 struct VerifyArgs {

@@ -24,7 +24,7 @@ using namespace tfsm_dev;
 // Search-kernel work splitting: record capacity per launch and the default
 // step budget after which a piece donates work.
 constexpr uint32_t kSplitCap = 1u << 21;
-constexpr uint32_t kDefaultSplitSteps = 4096;
+constexpr uint32_t kDefaultSplitSteps = 1024;
 
 namespace {
 
@@ -281,15 +281,27 @@ void make_plans(PatStatic &P, const uint32_t *slot_of /* [kMaxV*kMaxV] */) {
         }
         bool vis[kMaxV] = {false};
         vis[order[0]] = true;
+        uint32_t sec = 0;
         for (uint32_t d = 0; d < nv; ++d) {
             P.order[start][d] = order[d];
             P.bn_n[start][d] = 0;
+            P.sec_base[start][d] = (uint8_t)std::min(sec, 255u);
+            uint16_t before = 0;
+            uint8_t after = 0;
+            for (uint32_t e = 0; e < nv; ++e) {
+                if (e == d || P.vlab[order[e]] != P.vlab[order[d]]) continue;
+                if (e < d) before |= (uint16_t)(1u << e);
+                else after = 1;
+            }
+            P.same_before[start][d] = before;
+            P.same_after[start][d] = after;
             if (d == 0) continue;
             const uint32_t x = order[d];
             for (uint32_t j = 0; j < P.nbr_n[x]; ++j) {
                 const uint32_t y = P.nbr_v[x][j];
                 if (vis[y]) P.bn_slot[start][d][P.bn_n[start][d]++] = (uint8_t)slot_of[y * kMaxV + x];
             }
+            sec += P.bn_n[start][d] - 1;
             vis[x] = true;
         }
     }
@@ -562,8 +574,9 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         uint32_t max_nv = 0;
         for (uint32_t p = 0; p < n; ++p) if (decided[p] == TFSM_DECIDED_SEARCH) max_nv = std::max(max_nv, hs[p].nv);
         // persistent grid: resident blocks per CU (LDS-bound for the 16-deep variant)
+        // (LDS per block: 51 KB for the 8-deep variant, 100 KB for the 16-deep one)
         const uint32_t per_cu = opt.grid_waves_per_cu ? std::max(1u, opt.grid_waves_per_cu / (kBlock / 64))
-                                                      : (max_nv <= 8 ? 4u : 2u);
+                                                      : (max_nv <= 8 ? 3u : 1u);
         const uint32_t grid = (uint32_t)c->num_cus * per_cu;
         if (max_nv <= 8) hipLaunchKernelGGL(k_verify<8>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
         else hipLaunchKernelGGL(k_verify<16>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
