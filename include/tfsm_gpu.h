@@ -170,8 +170,46 @@ typedef struct {
     uint64_t search_bytes;
     uint32_t search_splits;     /* sub-tree pieces donated between lanes */
     uint32_t search_watchdog;   /* 1 when a drained wave left on its idle limit */
+    float rank_ms;              /* features, scores and priority sort before the search */
 } tfsm_call_stats;
 int tfsm_last_stats(tfsm_ctx *ctx, tfsm_call_stats *out);
+
+/* ---- Learned task ordering (no counterpart in the reference, which always
+ * refills tasks in u-ascending, domain order, comper.h:1181-1239).
+ *
+ * A 24 -> 64 (ReLU) -> 1 perceptron scores every (pattern, u, candidate)
+ * task from 24 integer-derived features; its output is a logit of
+ * P(candidate valid).  Weights are stored as BF16 on the device and every
+ * sum is accumulated in FP32.  With task_order = TFSM_ORDER_LEARNED the
+ * scores only reorder and abandon work: the existence search still decides
+ * every verdict, so results equal those of the reference order. */
+#define TFSM_RANKER_FEATURES 24
+#define TFSM_RANKER_HIDDEN 64
+
+typedef struct {
+    float w1[TFSM_RANKER_HIDDEN * TFSM_RANKER_FEATURES];   /* row-major [hidden][feature] */
+    float b1[TFSM_RANKER_HIDDEN];
+    float w2[TFSM_RANKER_HIDDEN];
+    float b2;
+} tfsm_ranker_weights;
+
+/* Install weights (rounded to BF16) / read back the installed BF16 weights. */
+int tfsm_ranker_set(tfsm_ctx *ctx, const tfsm_ranker_weights *w);
+int tfsm_ranker_get(tfsm_ctx *ctx, tfsm_ranker_weights *w);
+/* Logits of n feature rows (row-major n x 24) with the installed weights. */
+int tfsm_ranker_forward(tfsm_ctx *ctx, const float *x, uint64_t n, float *logits);
+/* Mean binary cross-entropy over n rows (labels 0/1) and its gradient with
+ * respect to the installed weights. */
+int tfsm_ranker_gradient(tfsm_ctx *ctx, const float *x, const float *y, uint64_t n, tfsm_ranker_weights *grad,
+                         float *loss);
+/* Adam on the GPU from the installed weights (FP32 master copy, BF16 device
+ * weights refreshed after each step); mini-batches drawn with a seeded
+ * shuffle.  Installs the result; *loss gets the last epoch's mean loss. */
+int tfsm_ranker_train(tfsm_ctx *ctx, const float *x, const float *y, uint64_t n, uint32_t epochs, uint32_t batch,
+                      float lr, uint64_t seed, float *loss);
+/* Features of the tasks of the last eval call made with opt.trace set, one
+ * row of 24 per task in reference task order (as tfsm_read_trace). */
+int64_t tfsm_read_features(tfsm_ctx *ctx, float *out, uint64_t cap_rows);
 
 const char *tfsm_status_string(int status);
 const char *tfsm_last_error(tfsm_ctx *ctx);

@@ -74,7 +74,6 @@ struct PatStatic {
     uint16_t same_before[kMaxV][kMaxV]; // earlier depths whose vertex has d's label
     uint8_t same_after[kMaxV][kMaxV];   // 1 when a later depth has d's label
     uint32_t task_prefix[kMaxV + 1];    // reference task order: u-major prefix of |C(u)|
-    const uint32_t *perm;               // optional task permutation (learned order)
 };
 
 // Device-written state of one pattern.
@@ -136,6 +135,8 @@ struct VerifyArgs {
     SearchCounters *ctr;
     const uint32_t *pool;
     uint32_t *trace;                       // nullable, indexed by reference task order
+    const uint32_t *order;                 // nullable: search order -> reference task index
+    uint32_t abort_stopped;                // capped mode: abandon tasks of a u that has freq valid
     uint32_t steps_per_round;
     uint32_t split_steps;                  // donate work after this many steps (0: never)
     SplitRecord *records;                  // split_cap records

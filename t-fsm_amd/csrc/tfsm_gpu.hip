@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -165,6 +166,11 @@ struct tfsm_ctx {
     uint32_t *d_trace = nullptr;
     uint64_t trace_len = 0, trace_cap = 0;
     unsigned long long *d_scanned = nullptr;
+    // ranker
+    uint16_t *d_rank_w = nullptr;            // installed BF16 parameters (kParams)
+    bool ranker_installed = false;
+    float *d_feat = nullptr;                 // features of the last traced call
+    uint64_t feat_len = 0, feat_cap = 0;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     tfsm_call_stats stats{};
     std::string err;
@@ -321,7 +327,10 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
     const tfsm_eval_options &opt = call.opt;
     if (opt.mode != TFSM_MODE_CAPPED && opt.mode != TFSM_MODE_EXACT)
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown mode"};
-    if (opt.task_order != TFSM_ORDER_REFERENCE)
+    if (opt.task_order != TFSM_ORDER_REFERENCE && opt.task_order != TFSM_ORDER_LEARNED)
+        throw ArgError{TFSM_ERR_ARGUMENT, "unknown task order"};
+    const bool learned = opt.task_order == TFSM_ORDER_LEARNED;
+    if (learned && !c->ranker_installed)
         throw ArgError{TFSM_ERR_ARGUMENT, "learned task order requires an installed ranker"};
     if (call.freq == 0) throw ArgError{TFSM_ERR_ARGUMENT, "freq must be at least 1"};
     for (uint32_t i = 0; i < n; ++i) {
@@ -568,6 +577,45 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         va.split_cap = kSplitCap;
         va.piece_cnt = c->d_piece_cnt;
         va.piece_incomplete = c->d_piece_inc;
+        va.order = nullptr;
+        va.abort_stopped = 0;
+        if (total_tasks >= 0xffffffffull) throw ArgError{TFSM_ERR_LIMIT, "more than 2^32 tasks in one call"};
+        // features (traced calls and the learned order), scores, priority sort
+        const uint32_t gt = grid_for(total_tasks, c->num_cus);
+        float *feat = nullptr;
+        if (opt.trace) {
+            if (total_tasks > c->feat_cap) {
+                if (c->d_feat) HIPX(hipFree(c->d_feat));
+                c->d_feat = nullptr;
+                c->feat_cap = std::max<unsigned long long>(total_tasks, 1 << 18);
+                HIPX(hipMalloc(&c->d_feat, c->feat_cap * kFeat * sizeof(float)));
+            }
+            c->feat_len = total_tasks;
+            feat = c->d_feat;
+        } else if (learned) {
+            feat = c->work.alloc<float>(total_tasks * kFeat);
+        }
+        if (feat) hipLaunchKernelGGL(k_features, dim3(gt), dim3(kBlock), 0, s, c->g, d_ps, d_st, va.task_base, n,
+                                     total_tasks, feat);
+        if (learned) {
+            float *logits = c->work.alloc<float>(total_tasks);
+            float *sums = c->work.alloc<float>((size_t)n * kMaxV);
+            uint32_t *keys = c->work.alloc<uint32_t>(total_tasks);
+            uint32_t *hist = c->work.alloc<uint32_t>(kPrioBins);
+            uint32_t *order = c->work.alloc<uint32_t>(total_tasks);
+            HIPX(hipMemsetAsync(sums, 0, (size_t)n * kMaxV * sizeof(float), s));
+            HIPX(hipMemsetAsync(hist, 0, kPrioBins * sizeof(uint32_t), s));
+            hipLaunchKernelGGL(k_mlp_forward, dim3(gt), dim3(kBlock), 0, s, feat, total_tasks, c->d_rank_w, logits);
+            hipLaunchKernelGGL(k_group_sums, dim3(gt), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, logits,
+                               sums);
+            hipLaunchKernelGGL(k_priority, dim3(gt), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, logits,
+                               sums, keys, hist);
+            hipLaunchKernelGGL(k_prio_scan, dim3(1), dim3(kBlock), 0, s, hist);
+            hipLaunchKernelGGL(k_prio_scatter, dim3(gt), dim3(kBlock), 0, s, keys, total_tasks, hist, order);
+            HIPX(hipGetLastError());
+            va.order = order;
+            va.abort_stopped = 1;
+        }
         const uint32_t epoch = ++c->epoch;
         HIPX(hipMemsetAsync(c->d_ctr, 0, sizeof(SearchCounters), s));
         HIPX(hipEventRecord(c->ev[4], s));
@@ -582,6 +630,9 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         else hipLaunchKernelGGL(k_verify<16>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
         HIPX(hipGetLastError());
         HIPX(hipEventRecord(c->ev[5], s));
+        if (opt.trace)
+            hipLaunchKernelGGL(k_trace_final, dim3(gt), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks,
+                               c->d_trace);
     }
     HIPX(hipEventRecord(c->ev[3], s));
     HIPX(hipMemcpyAsync(hst, d_st, n * sizeof(PatState), hipMemcpyDeviceToHost, s));
@@ -594,6 +645,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
     if (total_tasks) {
         HIPX(hipEventElapsedTime(&c->stats.search_ms, c->ev[4], c->ev[5]));
         c->stats.search_launches = 1;
+        HIPX(hipEventElapsedTime(&c->stats.rank_ms, c->ev[2], c->ev[4]));
         c->stats.search_splits = std::min(hctr->split_tail, kSplitCap);
         c->stats.search_watchdog = hctr->watchdog ? 1u : 0u;
     }
@@ -785,6 +837,7 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         HIPX(hipMemset(c->d_records, 0, (size_t)kSplitCap * sizeof(SplitRecord)));
         HIPX(hipMemset(c->d_piece_inc, 0, (size_t)kSplitCap * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_scanned, sizeof(unsigned long long)));
+        HIPX(hipMalloc(&c->d_rank_w, kParams * sizeof(uint16_t)));
         return TFSM_OK;
     });
     if (rc != TFSM_OK) {
@@ -810,6 +863,8 @@ int tfsm_ctx_destroy(tfsm_ctx *c) {
     (void)hipFree(c->d_piece_cnt);
     (void)hipFree(c->d_piece_inc);
     (void)hipFree(c->d_scanned);
+    (void)hipFree(c->d_rank_w);
+    (void)hipFree(c->d_feat);
     (void)hipFree(c->d_trace);
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -936,6 +991,172 @@ int tfsm_last_stats(tfsm_ctx *ctx, tfsm_call_stats *out) {
     if (!ctx || !out) return TFSM_ERR_ARGUMENT;
     *out = ctx->stats;
     return TFSM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ranker
+// ---------------------------------------------------------------------------
+static uint16_t host_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint32_t rounding = ((u >> 16) & 1u) + 0x7fffu;
+    return (uint16_t)((u + rounding) >> 16);
+}
+
+static float host_f32(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+static_assert(sizeof(tfsm_ranker_weights) == kParams * sizeof(float), "ranker layout");
+
+int tfsm_ranker_set(tfsm_ctx *ctx, const tfsm_ranker_weights *w) {
+    if (!ctx || !w) return TFSM_ERR_ARGUMENT;
+    return guarded(ctx, [&]() -> int {
+        const float *f = reinterpret_cast<const float *>(w);
+        std::vector<uint16_t> h(kParams);
+        for (int i = 0; i < kParams; ++i) {
+            if (!std::isfinite(f[i])) throw ArgError{TFSM_ERR_ARGUMENT, "ranker weight is not finite"};
+            h[i] = host_bf16(f[i]);
+        }
+        HIPX(hipMemcpy(ctx->d_rank_w, h.data(), kParams * sizeof(uint16_t), hipMemcpyHostToDevice));
+        ctx->ranker_installed = true;
+        return TFSM_OK;
+    });
+}
+
+int tfsm_ranker_get(tfsm_ctx *ctx, tfsm_ranker_weights *w) {
+    if (!ctx || !w) return TFSM_ERR_ARGUMENT;
+    if (!ctx->ranker_installed) return fail(ctx, TFSM_ERR_ARGUMENT, "no ranker installed");
+    return guarded(ctx, [&]() -> int {
+        std::vector<uint16_t> h(kParams);
+        HIPX(hipMemcpy(h.data(), ctx->d_rank_w, kParams * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        float *f = reinterpret_cast<float *>(w);
+        for (int i = 0; i < kParams; ++i) f[i] = host_f32(h[i]);
+        return TFSM_OK;
+    });
+}
+
+namespace {
+
+// Device copies of n feature rows and labels in the scratch arena.
+void stage_rows(tfsm_ctx *c, const float *x, const float *y, uint64_t n, float **dx, float **dy) {
+    c->work.reset();
+    *dx = c->work.alloc<float>(n * kFeat);
+    HIPX(hipMemcpy(*dx, x, n * kFeat * sizeof(float), hipMemcpyHostToDevice));
+    if (y) {
+        *dy = c->work.alloc<float>(n);
+        HIPX(hipMemcpy(*dy, y, n * sizeof(float), hipMemcpyHostToDevice));
+    }
+}
+
+}  // namespace
+
+int tfsm_ranker_forward(tfsm_ctx *ctx, const float *x, uint64_t n, float *logits) {
+    if (!ctx || (n && (!x || !logits))) return TFSM_ERR_ARGUMENT;
+    if (!ctx->ranker_installed) return fail(ctx, TFSM_ERR_ARGUMENT, "no ranker installed");
+    if (!n) return TFSM_OK;
+    return guarded(ctx, [&]() -> int {
+        float *dx, *dy = nullptr;
+        stage_rows(ctx, x, nullptr, n, &dx, &dy);
+        float *dl = ctx->work.alloc<float>(n);
+        hipLaunchKernelGGL(k_mlp_forward, dim3(grid_for(n, ctx->num_cus)), dim3(kBlock), 0, ctx->stream, dx, n,
+                           ctx->d_rank_w, dl);
+        HIPX(hipGetLastError());
+        HIPX(hipMemcpyAsync(logits, dl, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIPX(hipStreamSynchronize(ctx->stream));
+        return TFSM_OK;
+    });
+}
+
+int tfsm_ranker_gradient(tfsm_ctx *ctx, const float *x, const float *y, uint64_t n, tfsm_ranker_weights *grad,
+                         float *loss) {
+    if (!ctx || !grad || !n || !x || !y) return TFSM_ERR_ARGUMENT;
+    if (!ctx->ranker_installed) return fail(ctx, TFSM_ERR_ARGUMENT, "no ranker installed");
+    return guarded(ctx, [&]() -> int {
+        float *dx, *dy;
+        stage_rows(ctx, x, y, n, &dx, &dy);
+        float *dg = ctx->work.alloc<float>(kParams + 1);
+        HIPX(hipMemsetAsync(dg, 0, (kParams + 1) * sizeof(float), ctx->stream));
+        hipLaunchKernelGGL(k_mlp_backward, dim3(grid_for(n, ctx->num_cus)), dim3(kBlock), 0, ctx->stream, dx, dy,
+                           (const uint32_t *)nullptr, n, ctx->d_rank_w, dg, dg + kParams);
+        HIPX(hipGetLastError());
+        std::vector<float> h(kParams + 1);
+        HIPX(hipMemcpyAsync(h.data(), dg, (kParams + 1) * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIPX(hipStreamSynchronize(ctx->stream));
+        float *g = reinterpret_cast<float *>(grad);
+        for (int i = 0; i < kParams; ++i) g[i] = h[i] / (float)n;
+        if (loss) *loss = h[kParams] / (float)n;
+        return TFSM_OK;
+    });
+}
+
+int tfsm_ranker_train(tfsm_ctx *ctx, const float *x, const float *y, uint64_t n, uint32_t epochs, uint32_t batch,
+                      float lr, uint64_t seed, float *loss) {
+    if (!ctx || !x || !y || !n || !epochs || !batch || !(lr > 0.f)) return TFSM_ERR_ARGUMENT;
+    if (!ctx->ranker_installed) return fail(ctx, TFSM_ERR_ARGUMENT, "install initial weights first");
+    if (n >= 0xffffffffull) return fail(ctx, TFSM_ERR_LIMIT, "more than 2^32 training rows");
+    return guarded(ctx, [&]() -> int {
+        float *dx, *dy;
+        stage_rows(ctx, x, y, n, &dx, &dy);
+        float *master = ctx->work.alloc<float>(kParams);
+        float *m1 = ctx->work.alloc<float>(kParams);
+        float *m2 = ctx->work.alloc<float>(kParams);
+        float *dg = ctx->work.alloc<float>(kParams + 1);
+        uint32_t *didx = ctx->work.alloc<uint32_t>(n);
+        std::vector<uint16_t> hw(kParams);
+        std::vector<float> hm(kParams);
+        HIPX(hipMemcpy(hw.data(), ctx->d_rank_w, kParams * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        for (int i = 0; i < kParams; ++i) hm[i] = host_f32(hw[i]);
+        HIPX(hipMemcpy(master, hm.data(), kParams * sizeof(float), hipMemcpyHostToDevice));
+        HIPX(hipMemset(m1, 0, kParams * sizeof(float)));
+        HIPX(hipMemset(m2, 0, kParams * sizeof(float)));
+        std::vector<uint32_t> perm(n);
+        for (uint64_t i = 0; i < n; ++i) perm[i] = (uint32_t)i;
+        uint64_t state = seed ? seed : 0x9E3779B97F4A7C15ull;
+        auto next = [&]() {   // splitmix64
+            uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        };
+        uint32_t step = 0;
+        float last = 0.f;
+        for (uint32_t e = 0; e < epochs; ++e) {
+            for (uint64_t i = n - 1; i > 0; --i) std::swap(perm[i], perm[next() % (i + 1)]);
+            HIPX(hipMemcpyAsync(didx, perm.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+            float *eloss = ctx->work.alloc<float>(1);
+            HIPX(hipMemsetAsync(eloss, 0, sizeof(float), ctx->stream));
+            for (uint64_t b = 0; b < n; b += batch) {
+                const uint64_t nb = std::min<uint64_t>(batch, n - b);
+                HIPX(hipMemsetAsync(dg, 0, kParams * sizeof(float), ctx->stream));
+                hipLaunchKernelGGL(k_mlp_backward, dim3(grid_for(nb, ctx->num_cus)), dim3(kBlock), 0, ctx->stream,
+                                   dx, dy, didx + b, nb, ctx->d_rank_w, dg, eloss);
+                ++step;
+                hipLaunchKernelGGL(k_adam, dim3((kParams + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, master,
+                                   m1, m2, dg, 1.f / (float)nb, lr, 0.9f, 0.999f, 1e-8f, step, ctx->d_rank_w);
+                HIPX(hipGetLastError());
+            }
+            HIPX(hipMemcpyAsync(&last, eloss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+            HIPX(hipStreamSynchronize(ctx->stream));
+            last /= (float)n;
+        }
+        if (loss) *loss = last;
+        return TFSM_OK;
+    });
+}
+
+int64_t tfsm_read_features(tfsm_ctx *ctx, float *out, uint64_t cap_rows) {
+    if (!ctx) return TFSM_ERR_ARGUMENT;
+    const uint64_t n = ctx->feat_len;
+    int rc = guarded(ctx, [&]() -> int {
+        if (out && n)
+            HIPX(hipMemcpy(out, ctx->d_feat, std::min(n, cap_rows) * kFeat * sizeof(float), hipMemcpyDeviceToHost));
+        return TFSM_OK;
+    });
+    return rc == TFSM_OK ? (int64_t)n : rc;
 }
 
 }  // extern "C"

@@ -8,6 +8,7 @@
 #include "graph.h"
 #include "miner.h"
 #include "pattern.h"
+#include "ranker.h"
 
 extern "C" {
 
@@ -64,6 +65,8 @@ void tfsmh_edge_stats(const tfsmh_graph *h, int32_t *out) {
         out[k++] = (int32_t)e.na; out[k++] = (int32_t)e.nb;
     }
 }
+
+int tfsmh_load_ranker(tfsm_ctx *ctx, const char *path) { return tfsm::load_ranker_file(ctx, path); }
 
 int tfsmh_ctx_create(const tfsmh_graph *h, tfsm_ctx **out) {
     tfsm_csr c = tfsm::csr_view(h->g);

@@ -16,6 +16,7 @@
 
 #include "graph.h"
 #include "miner.h"
+#include "ranker.h"
 
 namespace {
 
@@ -63,8 +64,14 @@ int main(int argc, char **argv) {
         return 1;
     }
     if (ranker_path) {
-        std::cerr << "run: -learned is not available in this build\n";
-        return 2;
+        rc = tfsm::load_ranker_file(ctx, ranker_path);
+        if (rc != TFSM_OK) {
+            std::cerr << "run: cannot install ranker " << ranker_path << ": " << tfsm_status_string(rc) << " "
+                      << tfsm_last_error(ctx) << "\n";
+            tfsm_ctx_destroy(ctx);
+            return 1;
+        }
+        opt.learned = true;
     }
     auto t2 = clk::now();
     std::vector<tfsm::FoundPattern> found;

@@ -60,14 +60,38 @@ class CallStats(ctypes.Structure):
     _fields_ = [("filter_ms", ctypes.c_float), ("table_ms", ctypes.c_float), ("search_ms", ctypes.c_float),
                 ("search_launches", ctypes.c_uint32), ("table_bytes", ctypes.c_uint64),
                 ("search_bytes", ctypes.c_uint64), ("search_splits", ctypes.c_uint32),
-                ("search_watchdog", ctypes.c_uint32)]
+                ("search_watchdog", ctypes.c_uint32), ("rank_ms", ctypes.c_float)]
+
+
+RANKER_FEATURES = 24
+RANKER_HIDDEN = 64
+RANKER_PARAMS = RANKER_HIDDEN * RANKER_FEATURES + 2 * RANKER_HIDDEN + 1
+
+
+class RankerWeights(ctypes.Structure):
+    _fields_ = [("w1", ctypes.c_float * (RANKER_HIDDEN * RANKER_FEATURES)), ("b1", ctypes.c_float * RANKER_HIDDEN),
+                ("w2", ctypes.c_float * RANKER_HIDDEN), ("b2", ctypes.c_float)]
+
+    def to_numpy(self):
+        import numpy as np
+        return np.frombuffer(bytes(self), dtype=np.float32).copy()
+
+    @staticmethod
+    def from_numpy(flat):
+        import numpy as np
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        assert flat.size == RANKER_PARAMS
+        w = RankerWeights()
+        ctypes.memmove(ctypes.addressof(w), flat.ctypes.data, flat.nbytes)
+        return w
 
 
 # every symbol include/tfsm_gpu.h declares
 GPU_SYMBOLS = ["tfsm_ctx_create", "tfsm_ctx_destroy", "tfsm_eval_pattern", "tfsm_eval_level",
                "tfsm_domain_read", "tfsm_domain_release", "tfsm_last_domain", "tfsm_last_states",
                "tfsm_last_aux", "tfsm_last_order", "tfsm_read_trace", "tfsm_last_stats",
-               "tfsm_status_string", "tfsm_last_error"]
+               "tfsm_status_string", "tfsm_last_error", "tfsm_ranker_set", "tfsm_ranker_get",
+               "tfsm_ranker_forward", "tfsm_ranker_gradient", "tfsm_ranker_train", "tfsm_read_features"]
 
 _gpu = None
 _host = None
@@ -110,6 +134,15 @@ def gpu_lib():
         lib.tfsm_status_string.restype = ctypes.c_char_p
         lib.tfsm_last_error.argtypes = [vp]
         lib.tfsm_last_error.restype = ctypes.c_char_p
+        fp = P(ctypes.c_float)
+        lib.tfsm_ranker_set.argtypes = [vp, P(RankerWeights)]
+        lib.tfsm_ranker_get.argtypes = [vp, P(RankerWeights)]
+        lib.tfsm_ranker_forward.argtypes = [vp, fp, ctypes.c_uint64, fp]
+        lib.tfsm_ranker_gradient.argtypes = [vp, fp, fp, ctypes.c_uint64, P(RankerWeights), fp]
+        lib.tfsm_ranker_train.argtypes = [vp, fp, fp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_float, ctypes.c_uint64, fp]
+        lib.tfsm_read_features.argtypes = [vp, fp, ctypes.c_uint64]
+        lib.tfsm_read_features.restype = ctypes.c_int64
         _gpu = lib
     return _gpu
 
@@ -136,6 +169,7 @@ def host_lib():
         lib.tfsmh_extend.argtypes = [vp, ctypes.c_int, P(ctypes.c_int32), ctypes.c_int, P(ctypes.c_int32),
                                      ctypes.c_int]
         lib.tfsmh_last_error.restype = ctypes.c_char_p
+        lib.tfsmh_load_ranker.argtypes = [vp, ctypes.c_char_p]
         _host = lib
     return _host
 
@@ -222,8 +256,10 @@ class Context:
             raise TfsmError(rc, "tfsm_ctx_create")
         self.h = h
 
-    def eval_pattern(self, pat, freq, mode=MODE_CAPPED, parent=0, retain=False, trace=False, split_steps=0):
-        opt = EvalOptions(mode=mode, retain_domains=int(retain), task_order=ORDER_REFERENCE, trace=int(trace),
+    def eval_pattern(self, pat, freq, mode=MODE_CAPPED, parent=0, retain=False, trace=False, split_steps=0,
+                     learned=False):
+        opt = EvalOptions(mode=mode, retain_domains=int(retain),
+                          task_order=ORDER_LEARNED if learned else ORDER_REFERENCE, trace=int(trace),
                           split_steps=split_steps)
         res = PatternResult()
         _check(self.h, self.lib.tfsm_eval_pattern(self.h, ctypes.byref(pat), ctypes.c_uint64(parent), int(freq),
@@ -231,12 +267,13 @@ class Context:
         return res
 
     def eval_level(self, pats, freq, mode=MODE_CAPPED, parents=None, retain=False, trace=False, split_steps=0,
-                   grid_waves_per_cu=0):
+                   grid_waves_per_cu=0, learned=False):
         n = len(pats)
         arr = (Pattern * n)(*pats)
         par = (ctypes.c_uint64 * n)(*(parents or [0] * n))
         res = (PatternResult * n)()
-        opt = EvalOptions(mode=mode, retain_domains=int(retain), task_order=ORDER_REFERENCE, trace=int(trace),
+        opt = EvalOptions(mode=mode, retain_domains=int(retain),
+                          task_order=ORDER_LEARNED if learned else ORDER_REFERENCE, trace=int(trace),
                           split_steps=split_steps, grid_waves_per_cu=grid_waves_per_cu)
         _check(self.h, self.lib.tfsm_eval_level(self.h, arr, par, n, int(freq), ctypes.byref(opt), res))
         return list(res)
@@ -285,6 +322,54 @@ class Context:
         out = np.zeros(max(n, 1), np.uint32)
         self.lib.tfsm_read_trace(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n)
         return out[:n]
+
+    def read_features(self):
+        import numpy as np
+        n = self.lib.tfsm_read_features(self.h, None, 0)
+        out = np.zeros((max(n, 1), RANKER_FEATURES), np.float32)
+        self.lib.tfsm_read_features(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n)
+        return out[:n]
+
+    def ranker_load(self, path):
+        _check(self.h, host_lib().tfsmh_load_ranker(self.h, path.encode()))
+
+    def ranker_set(self, flat):
+        w = RankerWeights.from_numpy(flat)
+        _check(self.h, self.lib.tfsm_ranker_set(self.h, ctypes.byref(w)))
+
+    def ranker_get(self):
+        w = RankerWeights()
+        _check(self.h, self.lib.tfsm_ranker_get(self.h, ctypes.byref(w)))
+        return w.to_numpy()
+
+    def ranker_forward(self, X):
+        import numpy as np
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros(len(X), np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        _check(self.h, self.lib.tfsm_ranker_forward(self.h, X.ctypes.data_as(fp), len(X), out.ctypes.data_as(fp)))
+        return out
+
+    def ranker_gradient(self, X, y):
+        import numpy as np
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = RankerWeights()
+        loss = ctypes.c_float()
+        fp = ctypes.POINTER(ctypes.c_float)
+        _check(self.h, self.lib.tfsm_ranker_gradient(self.h, X.ctypes.data_as(fp), y.ctypes.data_as(fp), len(X),
+                                                     ctypes.byref(g), ctypes.byref(loss)))
+        return g.to_numpy(), loss.value
+
+    def ranker_train(self, X, y, epochs=4, batch=4096, lr=3e-3, seed=1):
+        import numpy as np
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = ctypes.c_float()
+        fp = ctypes.POINTER(ctypes.c_float)
+        _check(self.h, self.lib.tfsm_ranker_train(self.h, X.ctypes.data_as(fp), y.ctypes.data_as(fp), len(X), epochs,
+                                                  batch, lr, seed, ctypes.byref(loss)))
+        return loss.value
 
     def domain_read(self, handle, u):
         import numpy as np
