@@ -4,16 +4,17 @@ on one MI355X.
 
 One step = one whole mining run (size-2 level from the edge statistics, then
 every candidate pattern of every extension level evaluated on the GPU) with
-the graph already resident in HBM.  The batched path (one fused launch
-sequence per level) is the reported value; the plain path (one launch
-sequence per pattern) is timed beside it.  The CPU baseline is the
-reference's own parallel build (compiled from its sources into oracle/_ref),
-timed once on this host's cores.
+the graph already resident in HBM.  The batched path with the learned task
+order (one fused launch sequence per level, tasks ordered by the shipped
+ranker, t-fsm_amd/rankers/powerlaw.f32, trained on seeds 11 and 12 only) is
+the reported value; the plain path (one launch sequence per pattern,
+reference task order) and the batched path without the ranker are timed
+beside it.  The CPU baseline is the reference's own parallel build (compiled
+from its sources into oracle/_ref), timed once on this host's cores.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
 import argparse
-import hashlib
 import json
 import os
 import subprocess
@@ -26,6 +27,7 @@ sys.path.insert(0, os.path.join(REPO, "t-fsm_amd"))
 
 WORKLOAD = dict(name="powerlaw100k", seed=2, freq=9000, max_nodes=5, vertices=100000, edges=1100000)
 HBM_PEAK_GBPS = 8000.0
+RANKER = os.path.join(REPO, "t-fsm_amd", "rankers", "powerlaw.f32")
 
 
 def host_threads():
@@ -73,8 +75,8 @@ def cpu_baseline(lg, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -91,20 +93,21 @@ def main():
                     str(WORKLOAD["seed"])], check=True, stdout=subprocess.DEVNULL)
     graph = tfsm.Graph(lg, WORKLOAD["freq"])
     ctx = tfsm.Context(graph)             # graph now resident in HBM
+    ctx.ranker_load(RANKER)
     gold = golden_keys()
     threads = host_threads()
 
-    def run(batched, steps, warmup, tag):
+    def run(batched, learned, steps, warmup, tag):
         keys_path = os.path.join(tmp, f"keys_{tag}.txt")
         for _ in range(warmup):
-            ctx.mine(WORKLOAD["freq"], WORKLOAD["max_nodes"], threads=threads, batched=batched)
+            ctx.mine(WORKLOAD["freq"], WORKLOAD["max_nodes"], threads=threads, batched=batched, learned=learned)
         torch.cuda.synchronize()
         agg = dict(evaluated=0, frequent=0, verdicts=0, searches=0, search_ms=0.0, search_launches=0,
                    bytes_search=0, filter_ms=0.0, table_ms=0.0, bytes_table=0)
         t0 = time.perf_counter()
         for i in range(steps):
             n, st = ctx.mine(WORKLOAD["freq"], WORKLOAD["max_nodes"], threads=threads, batched=batched,
-                             keys_path=keys_path if i == 0 else None)
+                             learned=learned, keys_path=keys_path if i == 0 else None)
             for k in agg:
                 agg[k] += st[k]
         torch.cuda.synchronize()
@@ -113,13 +116,15 @@ def main():
         parity = gold is None or keys == gold
         return elapsed, agg, parity, n
 
-    plain_s, plain_agg, plain_ok, _ = run(False, args.steps, args.warmup, "plain")
-    batch_s, agg, batch_ok, nfreq = run(True, args.steps, args.warmup, "batched")
+    plain_s, plain_agg, plain_ok, _ = run(False, False, args.steps, args.warmup, "plain")
+    batch_s, batch_agg, batch_ok, _ = run(True, False, args.steps, args.warmup, "batched")
+    learn_s, agg, learn_ok, nfreq = run(True, True, args.steps, args.warmup, "learned")
 
     K = args.steps
     evaluated = agg["evaluated"] / K
-    value = agg["evaluated"] / batch_s
+    value = agg["evaluated"] / learn_s
     plain_value = plain_agg["evaluated"] / plain_s
+    batch_value = batch_agg["evaluated"] / batch_s
     ms_search = agg["search_ms"] / max(1, agg["search_launches"])
     bytes_per_launch = agg["bytes_search"] / max(1, agg["search_launches"])
     achieved = bytes_per_launch / (ms_search * 1e-3) / 1e9 if ms_search > 0 else 0.0
@@ -144,7 +149,7 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * batch_s / K, 3),
+        "ms_per_step": round(1e3 * learn_s / K, 3),
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
@@ -152,15 +157,25 @@ def main():
         "data": "synthetic (seeded Chung-Lu generator, t-fsm_amd/host/gen.h)",
         "config": {"workload": "powerlaw100k_chunglu_freq9000_maxnodes5", "vertices": WORKLOAD["vertices"],
                    "edges": WORKLOAD["edges"], "freq": WORKLOAD["freq"], "max_nodes": WORKLOAD["max_nodes"],
-                   "path": "batched", "host_threads": threads},
-        "verifications_per_s": round(agg["verdicts"] / batch_s, 1),
-        "frequent_patterns_per_s": round(nfreq * K / batch_s, 3),
+                   "path": "batched+learned", "ranker": "t-fsm_amd/rankers/powerlaw.f32 (trained on seeds 11, 12)",
+                   "host_threads": threads},
+        "verifications_per_s": round(agg["verdicts"] / learn_s, 1),
+        "frequent_patterns_per_s": round(nfreq * K / learn_s, 3),
         "patterns_evaluated_per_step": evaluated,
         "frequent_patterns": nfreq,
-        "pattern_set_matches_reference": bool(batch_ok and plain_ok),
+        "searches_per_step": agg["searches"] / K,
+        "pattern_set_matches_reference": bool(learn_ok and batch_ok and plain_ok),
         "plain": {"value": round(plain_value, 3), "ms_per_step": round(1e3 * plain_s / K, 3),
-                  "verifications_per_s": round(plain_agg["verdicts"] / plain_s, 1)},
+                  "verifications_per_s": round(plain_agg["verdicts"] / plain_s, 1),
+                  "searches_per_step": plain_agg["searches"] / K,
+                  "search_ms_per_step": round(plain_agg["search_ms"] / K, 3)},
+        "batched": {"value": round(batch_value, 3), "ms_per_step": round(1e3 * batch_s / K, 3),
+                    "verifications_per_s": round(batch_agg["verdicts"] / batch_s, 1),
+                    "searches_per_step": batch_agg["searches"] / K,
+                    "search_ms_per_step": round(batch_agg["search_ms"] / K, 3)},
+        "search_ms_per_step": round(agg["search_ms"] / K, 3),
         "speedup_vs_plain": round(value / plain_value, 3) if plain_value else None,
+        "speedup_batched_vs_plain": round(batch_value / plain_value, 3) if plain_value else None,
         "roofline": {"bound": "hbm", "kernel": "k_verify (existence search)", "achieved": round(achieved, 3),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 6),
                      "traffic": None, "launches": agg["search_launches"] // K,
