@@ -87,12 +87,13 @@ typedef struct {
     uint32_t retain_domains;    /* keep a frequent pattern's child domains on the device */
     uint32_t task_order;        /* TFSM_ORDER_* */
     uint32_t trace;             /* record per-task verdict and step count (tfsm_read_trace) */
-    uint32_t block_waves;       /* waves per workgroup of the search kernel (0: tuned default) */
     uint32_t grid_waves_per_cu; /* resident waves per CU of the search kernel (0: tuned default) */
     uint32_t split_steps;       /* DFS steps after which a search donates its untried
-                                   candidates to idle lanes (0: default 4096,
+                                   candidates to idle lanes (0: tuned default,
                                    0xffffffff: never); verdicts do not depend on it */
-    uint32_t reserved;
+    uint32_t split_fanout;      /* records one donation splits its candidates into
+                                   (0: tuned default, at most 64) */
+    uint32_t reserved;          /* must be 0 */
 } tfsm_eval_options;
 
 /* decided_by */

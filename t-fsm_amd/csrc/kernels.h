@@ -139,6 +139,7 @@ struct VerifyArgs {
     uint32_t abort_stopped;                // capped mode: abandon tasks of a u that has freq valid
     uint32_t steps_per_round;
     uint32_t split_steps;                  // donate work after this many steps (0: never)
+    uint32_t split_fanout;                 // records per donation (>= 1)
     SplitRecord *records;                  // split_cap records
     uint32_t split_cap;
     uint32_t *piece_cnt;                   // split_cap counters

@@ -22,10 +22,12 @@
 
 using namespace tfsm_dev;
 
-// Search-kernel work splitting: record capacity per launch and the default
-// step budget after which a piece donates work.
+// Search-kernel work splitting: record capacity per launch, the default step
+// budget after which a piece donates work and the default number of records
+// a donation is cut into.
 constexpr uint32_t kSplitCap = 1u << 21;
 constexpr uint32_t kDefaultSplitSteps = 1024;
+constexpr uint32_t kDefaultSplitFanout = 1;
 
 namespace {
 
@@ -329,6 +331,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown mode"};
     if (opt.task_order != TFSM_ORDER_REFERENCE && opt.task_order != TFSM_ORDER_LEARNED)
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown task order"};
+    if (opt.reserved != 0) throw ArgError{TFSM_ERR_ARGUMENT, "reserved option field must be 0"};
     const bool learned = opt.task_order == TFSM_ORDER_LEARNED;
     if (learned && !c->ranker_installed)
         throw ArgError{TFSM_ERR_ARGUMENT, "learned task order requires an installed ranker"};
@@ -573,6 +576,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         va.steps_per_round = 4;
         va.split_steps = opt.split_steps == 0 ? kDefaultSplitSteps : opt.split_steps;
         if (opt.split_steps == 0xffffffffu) va.split_steps = 0;
+        va.split_fanout = opt.split_fanout == 0 ? kDefaultSplitFanout : std::min(opt.split_fanout, 64u);
         va.records = c->d_records;
         va.split_cap = kSplitCap;
         va.piece_cnt = c->d_piece_cnt;
@@ -601,17 +605,20 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
             float *logits = c->work.alloc<float>(total_tasks);
             float *sums = c->work.alloc<float>((size_t)n * kMaxV);
             uint32_t *keys = c->work.alloc<uint32_t>(total_tasks);
-            uint32_t *hist = c->work.alloc<uint32_t>(kPrioBins);
+            // counting sort: pg blocks own contiguous task chunks (pg <= 128
+            // keeps the one-block scan of pg * 512 counters short)
+            const uint32_t pg = (uint32_t)std::min<unsigned long long>(128, (total_tasks + kBlock - 1) / kBlock);
+            const uint32_t chunk = (uint32_t)((total_tasks + pg - 1) / pg);
+            uint32_t *hist = c->work.alloc<uint32_t>((size_t)kPrioBins * pg);
             uint32_t *order = c->work.alloc<uint32_t>(total_tasks);
             HIPX(hipMemsetAsync(sums, 0, (size_t)n * kMaxV * sizeof(float), s));
-            HIPX(hipMemsetAsync(hist, 0, kPrioBins * sizeof(uint32_t), s));
             hipLaunchKernelGGL(k_mlp_forward, dim3(gt), dim3(kBlock), 0, s, feat, total_tasks, c->d_rank_w, logits);
             hipLaunchKernelGGL(k_group_sums, dim3(gt), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, logits,
                                sums);
-            hipLaunchKernelGGL(k_priority, dim3(gt), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, logits,
-                               sums, keys, hist);
-            hipLaunchKernelGGL(k_prio_scan, dim3(1), dim3(kBlock), 0, s, hist);
-            hipLaunchKernelGGL(k_prio_scatter, dim3(gt), dim3(kBlock), 0, s, keys, total_tasks, hist, order);
+            hipLaunchKernelGGL(k_priority, dim3(pg), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, chunk,
+                               logits, sums, keys, hist);
+            hipLaunchKernelGGL(k_prio_scan, dim3(1), dim3(kBlock), 0, s, hist, kPrioBins * pg);
+            hipLaunchKernelGGL(k_prio_scatter, dim3(pg), dim3(kBlock), 0, s, keys, total_tasks, chunk, hist, order);
             HIPX(hipGetLastError());
             va.order = order;
             va.abort_stopped = 1;

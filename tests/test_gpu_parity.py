@@ -173,23 +173,53 @@ def test_level_equals_single(workdir):
                 assert list(b.valid_count) == list(s.valid_count)
 
 
-@pytest.mark.parametrize("split_steps", [64, 0xffffffff])
-def test_split_search_parity(split_steps, workdir):
-    """Sub-tree donation at a tiny step budget (every 64 steps) and no donation
-    at all must both give the oracle's exact per-candidate verdicts."""
+NEVER = 0xffffffff
+
+
+@pytest.mark.parametrize("split_steps,fanout", [
+    (64, 1),           # record donation every 64 steps
+    (64, 16),          # each donation cut into 16 records
+    (NEVER, 1),        # no splitting at all
+    (0, 0),            # tuned defaults
+])
+def test_split_search_parity(split_steps, fanout, workdir):
+    """Sub-tree donation through the record queue at a tiny step budget, cut
+    into one or many records, and no splitting at all, must all give the
+    oracle's exact per-candidate verdicts."""
     env = env_for("powerlaw100k", 2, 9000, workdir)
+    opts = dict(split_steps=split_steps, split_fanout=fanout)
     for vl, es in variants(golden_keys("powerlaw100k_f9000_m5"), 15):
         pat = env.dense(vl, es)
         if pat is None:
             continue
-        res = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_EXACT, split_steps=split_steps)
+        res = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_EXACT, **opts)
         ref = env.og.eval(vl, es, env.freq, mode=1)
         assert (res.support_lower, res.support_upper) == (ref.lower, ref.upper), f"{vl} {es}"
         if ref.decided_by == 2:
             for u in range(len(vl)):
                 assert np.array_equal(env.ctx.last_states(u), ref.states(u)), f"{vl} {es} verdicts {u}"
-        cap = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_CAPPED, split_steps=split_steps)
+        cap = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_CAPPED, **opts)
         assert bool(cap.frequent) == ref.frequent
+
+
+@pytest.mark.skip(reason="an earlier form of this case (with work sharing inside a wave, since removed) made "
+                         "a GPU memory fault; it stays off the GPU until the exact-mode level path with "
+                         "split_steps=64 and fanout 4 is shown safe on the CPU")
+def test_split_level_parity(workdir):
+    """A whole level in one launch with aggressive splitting: exact supports
+    and verdict counts equal the per-pattern evaluations without splitting."""
+    env = env_for("powerlaw100k", 2, 9000, workdir)
+    pats = []
+    for vl, es in variants(golden_keys("powerlaw100k_f9000_m5"), 40):
+        pat = env.dense(vl, es)
+        if pat is not None and len(vl) >= 4:
+            pats.append(pat)
+    assert pats
+    lev = env.ctx.eval_level(pats, env.freq, mode=tfsm.MODE_EXACT, split_steps=64, split_fanout=4)
+    for pat, r in zip(pats, lev):
+        s = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_EXACT, split_steps=NEVER)
+        assert (r.support_lower, r.support_upper) == (s.support_lower, s.support_upper)
+        assert list(r.valid_count) == list(s.valid_count) and list(r.invalid_count) == list(s.invalid_count)
 
 
 def test_large_pattern_and_errors(workdir):
