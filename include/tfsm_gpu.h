@@ -93,7 +93,9 @@ typedef struct {
                                    0xffffffff: never); verdicts do not depend on it */
     uint32_t split_fanout;      /* records one donation splits its candidates into
                                    (0: tuned default, at most 64) */
-    uint32_t reserved;          /* must be 0 */
+    uint32_t long_steps;        /* DFS steps after which a root search moves to the
+                                   wave-cooperative search (0: tuned default,
+                                   0xffffffff: never); verdicts do not depend on it */
 } tfsm_eval_options;
 
 /* decided_by */

@@ -124,6 +124,10 @@ struct SearchCounters {
     uint32_t cnt_alloc;                    // piece counters handed out
     uint32_t watchdog;                     // set when a wave left on its idle limit
     uint32_t pad4[30];
+    uint32_t long_cnt;                     // root tasks handed to the wave-cooperative search
+    uint32_t pad5[31];
+    uint32_t long_next;                    // next of those taken by a wave
+    uint32_t pad6[31];
 };
 
 struct VerifyArgs {
@@ -144,6 +148,9 @@ struct VerifyArgs {
     uint32_t split_cap;
     uint32_t *piece_cnt;                   // split_cap counters
     uint32_t *piece_incomplete;            // split_cap flags
+    uint32_t long_steps;                   // root tasks still running after this many steps move on
+    uint32_t long_cap;                     // capacity of long_list (0: no wave-cooperative search)
+    uint32_t *long_list;                   // reference task indices for the wave-cooperative search
 };
 
 }  // namespace tfsm_dev

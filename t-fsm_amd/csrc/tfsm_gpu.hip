@@ -28,6 +28,11 @@ using namespace tfsm_dev;
 constexpr uint32_t kSplitCap = 1u << 21;
 constexpr uint32_t kDefaultSplitSteps = 1024;
 constexpr uint32_t kDefaultSplitFanout = 1;
+// Wave-cooperative second phase: default step budget of a root task in the
+// first phase, list capacity and workgroups per CU.
+constexpr uint32_t kDefaultLongSteps = 256;
+constexpr unsigned long long kLongCap = 1ull << 20;
+constexpr uint32_t kLongBlocksPerCu = 8;
 
 namespace {
 
@@ -331,7 +336,6 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown mode"};
     if (opt.task_order != TFSM_ORDER_REFERENCE && opt.task_order != TFSM_ORDER_LEARNED)
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown task order"};
-    if (opt.reserved != 0) throw ArgError{TFSM_ERR_ARGUMENT, "reserved option field must be 0"};
     const bool learned = opt.task_order == TFSM_ORDER_LEARNED;
     if (learned && !c->ranker_installed)
         throw ArgError{TFSM_ERR_ARGUMENT, "learned task order requires an installed ranker"};
@@ -633,9 +637,23 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         const uint32_t per_cu = opt.grid_waves_per_cu ? std::max(1u, opt.grid_waves_per_cu / (kBlock / 64))
                                                       : (max_nv <= 8 ? 3u : 1u);
         const uint32_t grid = (uint32_t)c->num_cus * per_cu;
+        // root tasks still running after long_steps steps finish in the
+        // wave-cooperative search (patterns of up to 8 vertices)
+        va.long_cap = 0;
+        va.long_list = nullptr;
+        va.long_steps = opt.long_steps == 0 ? kDefaultLongSteps : opt.long_steps;
+        if (opt.long_steps != 0xffffffffu && max_nv <= 8) {
+            va.long_cap = (uint32_t)std::min<unsigned long long>(total_tasks, kLongCap);
+            va.long_list = c->work.alloc<uint32_t>(va.long_cap);
+        }
         if (max_nv <= 8) hipLaunchKernelGGL(k_verify<8>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
         else hipLaunchKernelGGL(k_verify<16>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
         HIPX(hipGetLastError());
+        if (va.long_cap) {
+            hipLaunchKernelGGL(k_verify_long<8>, dim3((uint32_t)c->num_cus * kLongBlocksPerCu), dim3(kBlock), 0, s,
+                               va);
+            HIPX(hipGetLastError());
+        }
         HIPX(hipEventRecord(c->ev[5], s));
         if (opt.trace)
             hipLaunchKernelGGL(k_trace_final, dim3(gt), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks,

@@ -1,0 +1,132 @@
+// Wave-cooperative existence search: one (u, v) task per wave of 64 lanes.
+//
+// The same question as the lane-per-task search (is there an injective
+// embedding with u -> v? reference gmatch.h:1198-1280) with the work of each
+// DFS node spread over the wave: the candidates of depth d are the entries of
+// the shortest backward-neighbour row that occur in every other
+// backward-neighbour row (generateValidCandidateIndex, gmatch.h:1508-1546);
+// the wave tests 64 consecutive entries of that row at once (one lane each:
+// membership in the other rows by binary search, then injectivity against
+// earlier vertices with the same label), keeps the passing ones as a 64-bit
+// mask, and descends into them one at a time in ascending order.  For long
+// searches this divides the sequential chain of dependent loads by up to 64.
+//
+// The search state is uniform across the wave and lives in a WaveStack; every
+// lane runs the uniform code with identical values, and only the per-candidate
+// test differs by lane.  The code is generic over the wave (W: ballot over the
+// 64 lanes) and the data accessors (A), so that the same source runs on the
+// device and in a host check against the CPU oracle.  The includer defines
+// TFSM_WAVE_FN (the function qualifiers for its side).
+#pragma once
+
+#include <cstdint>
+
+namespace tfsm_wave {
+
+constexpr uint32_t kLanes = 64;
+
+enum WaveResult : uint32_t { kWaveFound = 1, kWaveExhausted = 2, kWaveAborted = 3 };
+
+template <int MV>
+struct WaveStack {
+    uint32_t idx[MV];                // candidate index per pattern vertex
+    uint32_t vid[MV];                // data vertex per depth
+    uint32_t base[MV];               // next untested position of depth d's primary row
+    uint32_t end[MV];                // end of depth d's primary row
+    uint32_t chunk[MV];              // first position of the chunk `mask` refers to
+    uint32_t prim[MV];               // which backward-neighbour row is primary
+    unsigned long long mask[MV];     // passing candidates of the chunk not yet entered
+    uint32_t rlo[MV][MV], rhi[MV][MV];   // backward-neighbour rows of depth d
+};
+
+// Index of the lowest set bit (m != 0).
+TFSM_WAVE_FN uint32_t wave_ctz64(unsigned long long m) { return (uint32_t)__builtin_ctzll(m); }
+
+// Is c in the ascending run a.pool(lo .. hi-1)?
+template <class A>
+TFSM_WAVE_FN bool wave_contains(const A &a, uint32_t lo, uint32_t hi, uint32_t c) {
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        const uint32_t x = a.pool(mid);
+        if (x == c) return true;
+        if (x < c) lo = mid + 1; else hi = mid;
+    }
+    return false;
+}
+
+// Rows of depth d (the candidate indices of its backward neighbours' images)
+// and the primary (shortest) one.
+template <int MV, class A>
+TFSM_WAVE_FN void wave_enter(const A &a, uint32_t u, uint32_t d, WaveStack<MV> &s) {
+    const uint32_t nb = a.bn_n(u, d);
+    uint32_t best = 0xffffffffu, prim = 0;
+    for (uint32_t k = 0; k < nb; ++k) {
+        uint32_t lo, hi;
+        a.row(u, d, k, s.idx[a.bn_vertex(u, d, k)], lo, hi);
+        s.rlo[d][k] = lo;
+        s.rhi[d][k] = hi;
+        if (hi - lo < best) {
+            best = hi - lo;
+            prim = k;
+        }
+    }
+    s.prim[d] = prim;
+    s.base[d] = s.rlo[d][prim];
+    s.end[d] = s.rhi[d][prim];
+    s.mask[d] = 0;
+}
+
+// Decides whether u -> candidate ci extends to a full embedding.  On
+// kWaveFound, s.idx holds the embedding (candidate index per pattern vertex).
+// a.abort(u, ci) is polled every `abort_every` rounds (0: never).
+template <int MV, class W, class A>
+TFSM_WAVE_FN uint32_t wave_exists(const W &w, const A &a, uint32_t u, uint32_t ci, WaveStack<MV> &s,
+                                  uint32_t abort_every) {
+    const uint32_t n = a.nv();
+    s.idx[u] = ci;
+    s.vid[0] = a.vid(u, ci);
+    if (n == 1) return kWaveFound;
+    uint32_t d = 1;
+    wave_enter<MV>(a, u, d, s);
+    for (uint32_t round = 1;; ++round) {
+        if (abort_every && round % abort_every == 0 && a.abort(u, ci)) return kWaveAborted;
+        if (s.mask[d] == 0) {
+            const uint32_t b0 = s.base[d], e = s.end[d];
+            if (b0 >= e) {
+                if (--d == 0) return kWaveExhausted;
+                continue;
+            }
+            // test the next 64 entries of the primary row, one per lane
+            const uint32_t ud = a.order(u, d), nb = a.bn_n(u, d), prim = s.prim[d];
+            const uint32_t before = a.same_before(u, d);
+            s.mask[d] = w.ballot([&](uint32_t lane) -> bool {
+                if (lane >= e - b0) return false;
+                const uint32_t c = a.pool(b0 + lane);
+                for (uint32_t k = 0; k < nb; ++k)
+                    if (k != prim && !wave_contains(a, s.rlo[d][k], s.rhi[d][k], c)) return false;
+                if (before) {
+                    const uint32_t v = a.vid(ud, c);
+                    for (uint32_t m = before; m; m &= m - 1)
+                        if (s.vid[wave_ctz64(m)] == v) return false;
+                }
+                return true;
+            });
+            s.chunk[d] = b0;
+            s.base[d] = e - b0 > kLanes ? b0 + kLanes : e;
+            continue;
+        }
+        // enter the lowest passing candidate of the chunk
+        const unsigned long long m = s.mask[d];
+        const uint32_t i = wave_ctz64(m);
+        s.mask[d] = m & (m - 1);
+        const uint32_t ud = a.order(u, d);
+        const uint32_t c = a.pool(s.chunk[d] + i);
+        s.idx[ud] = c;
+        s.vid[d] = a.vid(ud, c);
+        if (d + 1 == n) return kWaveFound;
+        ++d;
+        wave_enter<MV>(a, u, d, s);
+    }
+}
+
+}  // namespace tfsm_wave

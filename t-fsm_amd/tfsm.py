@@ -45,7 +45,7 @@ class Pattern(ctypes.Structure):
 class EvalOptions(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_uint32), ("retain_domains", ctypes.c_uint32), ("task_order", ctypes.c_uint32),
                 ("trace", ctypes.c_uint32), ("grid_waves_per_cu", ctypes.c_uint32), ("split_steps", ctypes.c_uint32),
-                ("split_fanout", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("split_fanout", ctypes.c_uint32), ("long_steps", ctypes.c_uint32)]
 
 
 class PatternResult(ctypes.Structure):
@@ -257,24 +257,26 @@ class Context:
         self.h = h
 
     def eval_pattern(self, pat, freq, mode=MODE_CAPPED, parent=0, retain=False, trace=False, split_steps=0,
-                     learned=False, split_fanout=0, grid_waves_per_cu=0):
+                     learned=False, split_fanout=0, grid_waves_per_cu=0, long_steps=0):
         opt = EvalOptions(mode=mode, retain_domains=int(retain),
                           task_order=ORDER_LEARNED if learned else ORDER_REFERENCE, trace=int(trace),
-                          split_steps=split_steps, split_fanout=split_fanout, grid_waves_per_cu=grid_waves_per_cu)
+                          split_steps=split_steps, split_fanout=split_fanout, grid_waves_per_cu=grid_waves_per_cu,
+                          long_steps=long_steps)
         res = PatternResult()
         _check(self.h, self.lib.tfsm_eval_pattern(self.h, ctypes.byref(pat), ctypes.c_uint64(parent), int(freq),
                                                   ctypes.byref(opt), ctypes.byref(res)))
         return res
 
     def eval_level(self, pats, freq, mode=MODE_CAPPED, parents=None, retain=False, trace=False, split_steps=0,
-                   grid_waves_per_cu=0, learned=False, split_fanout=0):
+                   grid_waves_per_cu=0, learned=False, split_fanout=0, long_steps=0):
         n = len(pats)
         arr = (Pattern * n)(*pats)
         par = (ctypes.c_uint64 * n)(*(parents or [0] * n))
         res = (PatternResult * n)()
         opt = EvalOptions(mode=mode, retain_domains=int(retain),
                           task_order=ORDER_LEARNED if learned else ORDER_REFERENCE, trace=int(trace),
-                          split_steps=split_steps, grid_waves_per_cu=grid_waves_per_cu, split_fanout=split_fanout)
+                          split_steps=split_steps, grid_waves_per_cu=grid_waves_per_cu, split_fanout=split_fanout,
+                          long_steps=long_steps)
         _check(self.h, self.lib.tfsm_eval_level(self.h, arr, par, n, int(freq), ctypes.byref(opt), res))
         return list(res)
 

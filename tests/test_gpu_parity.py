@@ -202,6 +202,30 @@ def test_split_search_parity(split_steps, fanout, workdir):
         assert bool(cap.frequent) == ref.frequent
 
 
+@pytest.mark.parametrize("long_steps", [64, 256, NEVER])
+@pytest.mark.parametrize("name,seed,freq,golden,limit", [
+    ("gnm2k", 1, 40, "gnm2k_f40_m4", 60),
+    ("powerlaw100k", 2, 9000, "powerlaw100k_f9000_m5", 40),
+])
+def test_long_search_parity(long_steps, name, seed, freq, golden, limit, workdir):
+    """Root searches handed to the wave-cooperative second phase after 64 or
+    256 steps, or never: the oracle's exact per-candidate verdicts and the
+    capped verdicts either way."""
+    env = env_for(name, seed, freq, workdir)
+    for vl, es in variants(golden_keys(golden), limit):
+        pat = env.dense(vl, es)
+        if pat is None:
+            continue
+        res = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_EXACT, long_steps=long_steps)
+        ref = env.og.eval(vl, es, env.freq, mode=1)
+        assert (res.support_lower, res.support_upper) == (ref.lower, ref.upper), f"{vl} {es}"
+        if ref.decided_by == 2:
+            for u in range(len(vl)):
+                assert np.array_equal(env.ctx.last_states(u), ref.states(u)), f"{vl} {es} verdicts {u}"
+        cap = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_CAPPED, long_steps=long_steps)
+        assert bool(cap.frequent) == ref.frequent
+
+
 @pytest.mark.skip(reason="an earlier form of this case (with work sharing inside a wave, since removed) made "
                          "a GPU memory fault; it stays off the GPU until the exact-mode level path with "
                          "split_steps=64 and fanout 4 is shown safe on the CPU")

@@ -1,0 +1,195 @@
+// Host check of the wave-cooperative existence search
+// (t-fsm_amd/csrc/wave_dfs.h): the same source, run with an emulated wave
+// (a loop over 64 lanes) on the CPU oracle's filtered domains, auxiliary
+// tables and matching orders, must reproduce the oracle's exact-mode verdict
+// of every candidate it is given.  Every memory access the search makes goes
+// through a bounds-checked accessor; a violation exits with status 2.
+//
+//   wave_dfs_check SPEC     (SPEC: see the pattern file written by
+//                            tests/test_wave_dfs.py)
+#define TFSM_WAVE_FN inline
+#include "../t-fsm_amd/csrc/wave_dfs.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../oracle/oracle.h"
+
+namespace {
+
+[[noreturn]] void violation(const char *what, uint64_t a, uint64_t b) {
+    std::fprintf(stderr, "bounds violation: %s (%llu vs %llu)\n", what, (unsigned long long)a,
+                 (unsigned long long)b);
+    std::exit(2);
+}
+
+struct HostWave {
+    template <class F>
+    unsigned long long ballot(F f) const {
+        unsigned long long m = 0;
+        for (uint32_t l = 0; l < tfsm_wave::kLanes; ++l)
+            if (f(l)) m |= 1ull << l;
+        return m;
+    }
+};
+
+struct HostAcc {
+    uint32_t n = 0;
+    std::vector<std::vector<uint32_t>> dom;                      // C(x), sorted data vertex ids
+    std::vector<uint32_t> pool_;                                 // every table, back to back
+    std::vector<std::vector<std::vector<uint32_t>>> off;         // off[b][x]: row starts over C(b), absolute
+    std::vector<std::vector<uint32_t>> ord;                      // ord[u][d]
+    std::vector<std::vector<std::vector<uint32_t>>> bn;          // bn[u][d]: backward neighbours of depth d
+    std::vector<std::vector<uint32_t>> sb;                       // sb[u][d]: earlier depths with d's label
+
+    uint32_t nv() const { return n; }
+    uint32_t order(uint32_t u, uint32_t d) const {
+        if (u >= n || d >= n) violation("order", u, d);
+        return ord[u][d];
+    }
+    uint32_t bn_n(uint32_t u, uint32_t d) const {
+        if (u >= n || d >= n) violation("bn_n", u, d);
+        return (uint32_t)bn[u][d].size();
+    }
+    uint32_t bn_vertex(uint32_t u, uint32_t d, uint32_t k) const {
+        if (k >= bn_n(u, d)) violation("bn_vertex", k, bn[u][d].size());
+        return bn[u][d][k];
+    }
+    void row(uint32_t u, uint32_t d, uint32_t k, uint32_t idx_b, uint32_t &lo, uint32_t &hi) const {
+        const uint32_t b = bn_vertex(u, d, k), x = order(u, d);
+        const std::vector<uint32_t> &o = off[b][x];
+        if (o.empty()) violation("row: no table", b, x);
+        if ((size_t)idx_b + 1 >= o.size()) violation("row index", idx_b, o.size());
+        lo = o[idx_b];
+        hi = o[idx_b + 1];
+        if (lo > hi || hi > pool_.size()) violation("row range", hi, pool_.size());
+    }
+    uint32_t pool(uint32_t pos) const {
+        if (pos >= pool_.size()) violation("pool", pos, pool_.size());
+        return pool_[pos];
+    }
+    uint32_t vid(uint32_t x, uint32_t c) const {
+        if (x >= n || c >= dom[x].size()) violation("vid", c, x < n ? dom[x].size() : 0);
+        return dom[x][c];
+    }
+    uint32_t same_before(uint32_t u, uint32_t d) const {
+        if (u >= n || d >= n) violation("same_before", u, d);
+        return sb[u][d];
+    }
+    bool abort(uint32_t, uint32_t) const { return false; }
+};
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    if (argc != 2) {
+        std::fprintf(stderr, "usage: wave_dfs_check SPEC\n");
+        return 64;
+    }
+    std::ifstream in(argv[1]);
+    std::string graph_path;
+    int freq = 0;
+    uint32_t samples = 0;
+    in >> graph_path >> freq >> samples;
+    orc_graph *g = orc_load(graph_path.c_str(), freq);
+    if (!g) {
+        std::fprintf(stderr, "cannot load %s\n", graph_path.c_str());
+        return 64;
+    }
+    std::string line;
+    std::getline(in, line);
+    uint64_t checked = 0, mismatches = 0, patterns = 0;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        int nv, ne;
+        if (!(ls >> nv)) continue;
+        std::vector<int32_t> vl(nv);
+        for (auto &x : vl) ls >> x;
+        ls >> ne;
+        std::vector<int32_t> ed(3 * ne);
+        for (auto &x : ed) ls >> x;
+        orc_eval *e = orc_eval_pattern(g, nv, vl.data(), ne, ed.data(), 0, nullptr, nullptr, freq, 1);
+        if (!e) continue;
+        if (orc_eval_decided_by(e) != 2 || nv > 16) {
+            orc_eval_free(e);
+            continue;
+        }
+        ++patterns;
+        HostAcc a;
+        a.n = (uint32_t)nv;
+        a.dom.resize(nv);
+        for (int x = 0; x < nv; ++x) {
+            const uint32_t sz = orc_eval_domain(e, x, nullptr, 0);
+            a.dom[x].resize(sz);
+            orc_eval_domain(e, x, a.dom[x].data(), sz);
+        }
+        a.off.assign(nv, std::vector<std::vector<uint32_t>>(nv));
+        for (int b = 0; b < nv; ++b)
+            for (int x = 0; x < nv; ++x) {
+                const int64_t cnt = orc_eval_aux(e, b, x, nullptr, nullptr);
+                if (cnt < 0) continue;
+                std::vector<uint32_t> o(a.dom[b].size() + 1), ix((size_t)cnt);
+                orc_eval_aux(e, b, x, o.data(), ix.data());
+                const uint32_t base = (uint32_t)a.pool_.size();
+                a.pool_.insert(a.pool_.end(), ix.begin(), ix.end());
+                for (auto &v : o) v += base;
+                a.off[b][x] = std::move(o);
+            }
+        std::vector<std::vector<int>> adj(nv, std::vector<int>(nv, 0));
+        for (int i = 0; i < ne; ++i) adj[ed[3 * i]][ed[3 * i + 1]] = adj[ed[3 * i + 1]][ed[3 * i]] = 1;
+        a.ord.assign(nv, std::vector<uint32_t>(nv));
+        a.bn.assign(nv, std::vector<std::vector<uint32_t>>(nv));
+        a.sb.assign(nv, std::vector<uint32_t>(nv, 0));
+        for (int u = 0; u < nv; ++u) {
+            orc_eval_order(e, u, a.ord[u].data());
+            for (int d = 0; d < nv; ++d)
+                for (int j = 0; j < d; ++j) {
+                    if (adj[a.ord[u][j]][a.ord[u][d]]) a.bn[u][d].push_back(a.ord[u][j]);
+                    if (vl[a.ord[u][j]] == vl[a.ord[u][d]]) a.sb[u][d] |= 1u << j;
+                }
+        }
+        tfsm_wave::WaveStack<16> st;
+        HostWave w;
+        for (int u = 0; u < nv; ++u) {
+            std::vector<uint8_t> states(a.dom[u].size());
+            orc_eval_states(e, u, states.data(), (uint32_t)states.size());
+            const size_t sz = states.size();
+            const size_t step = samples && sz > samples ? sz / samples : 1;
+            for (size_t ci = 0; ci < sz; ci += step) {
+                const uint32_t r = tfsm_wave::wave_exists<16>(w, a, (uint32_t)u, (uint32_t)ci, st, 0);
+                const uint8_t want = states[ci];
+                const uint8_t got = r == tfsm_wave::kWaveFound ? 1 : r == tfsm_wave::kWaveExhausted ? 2 : 0;
+                ++checked;
+                if (got != want) {
+                    ++mismatches;
+                    if (mismatches <= 10)
+                        std::fprintf(stderr, "mismatch: pattern %llu u %d ci %zu got %d want %d\n",
+                                     (unsigned long long)patterns, u, ci, got, want);
+                }
+                if (got == 1) {
+                    // the embedding is injective, edge-consistent and maps u to ci
+                    if (st.idx[u] != ci) violation("embedding root", st.idx[u], ci);
+                    for (int i = 0; i < nv; ++i)
+                        for (int j = i + 1; j < nv; ++j)
+                            if (a.dom[i][st.idx[i]] == a.dom[j][st.idx[j]]) violation("not injective", i, j);
+                    for (int k = 0; k < ne; ++k) {
+                        const uint32_t pa = ed[3 * k], pb = ed[3 * k + 1];
+                        const std::vector<uint32_t> &o = a.off[pa][pb];
+                        bool hit = false;
+                        for (uint32_t q = o[st.idx[pa]]; q < o[st.idx[pa] + 1]; ++q) hit |= a.pool_[q] == st.idx[pb];
+                        if (!hit) violation("embedding edge missing", pa, pb);
+                    }
+                }
+            }
+        }
+        orc_eval_free(e);
+    }
+    orc_graph_free(g);
+    std::printf("patterns %llu checked %llu mismatches %llu\n", (unsigned long long)patterns,
+                (unsigned long long)checked, (unsigned long long)mismatches);
+    return mismatches ? 1 : 0;
+}
