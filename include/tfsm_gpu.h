@@ -94,8 +94,13 @@ typedef struct {
     uint32_t split_fanout;      /* records one donation splits its candidates into
                                    (0: tuned default, at most 64) */
     uint32_t long_steps;        /* DFS steps after which a root search moves to the
-                                   wave-cooperative search (0: tuned default,
-                                   0xffffffff: never); verdicts do not depend on it */
+                                   wave-cooperative search (0: default, which is
+                                   never; 0xffffffff: never); verdicts do not depend on it */
+    uint32_t long_waves_per_cu; /* waves per CU of the wave-cooperative search (0: tuned default) */
+    uint32_t long_donate_every; /* its rounds between offers of work to idle waves
+                                   (0: tuned default, 0xffffffff: never) */
+    uint32_t long_fanout;       /* records per offer (0: tuned default, at most 64) */
+    uint32_t reserved;          /* must be 0 */
 } tfsm_eval_options;
 
 /* decided_by */

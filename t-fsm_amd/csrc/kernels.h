@@ -106,7 +106,9 @@ struct SplitRecord {
     uint32_t p, u, ci, floor, cnt_id, lo, hi;
     uint32_t prim;                         // which backward-neighbour row [lo, hi) indexes
     uint32_t prefix[kMaxV];
-    uint32_t pad[7];
+    uint32_t cpos;                         // wave-cooperative pieces: tested candidates at
+    uint32_t cmask_lo, cmask_hi;           //   positions cpos + i, i a set bit of the mask
+    uint32_t pad[4];
 };
 
 // Global bookkeeping of one search launch (zeroed before each launch).
@@ -128,6 +130,8 @@ struct SearchCounters {
     uint32_t pad5[31];
     uint32_t long_next;                    // next of those taken by a wave
     uint32_t pad6[31];
+    uint32_t long_busy;                    // second-phase waves with a task or piece
+    uint32_t pad7[31];
 };
 
 struct VerifyArgs {
@@ -151,6 +155,8 @@ struct VerifyArgs {
     uint32_t long_steps;                   // root tasks still running after this many steps move on
     uint32_t long_cap;                     // capacity of long_list (0: no wave-cooperative search)
     uint32_t *long_list;                   // reference task indices for the wave-cooperative search
+    uint32_t long_donate_every;            // its rounds between offers of work (0: never)
+    uint32_t long_fanout;                  // records per offer (>= 1)
 };
 
 }  // namespace tfsm_dev

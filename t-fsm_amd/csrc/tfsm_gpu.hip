@@ -29,10 +29,14 @@ constexpr uint32_t kSplitCap = 1u << 21;
 constexpr uint32_t kDefaultSplitSteps = 1024;
 constexpr uint32_t kDefaultSplitFanout = 1;
 // Wave-cooperative second phase: default step budget of a root task in the
-// first phase, list capacity and workgroups per CU.
-constexpr uint32_t kDefaultLongSteps = 256;
+// first phase (off by default: on the power-law workloads its rounds, binary
+// searches in global memory, cost more than the lane-per-task steps they
+// replace), list capacity, waves per CU, offer interval and fanout.
+constexpr uint32_t kDefaultLongSteps = 0xffffffffu;
 constexpr unsigned long long kLongCap = 1ull << 20;
-constexpr uint32_t kLongBlocksPerCu = 8;
+constexpr uint32_t kDefaultLongWavesPerCu = 8;
+constexpr uint32_t kDefaultLongDonateEvery = 32;
+constexpr uint32_t kDefaultLongFanout = 4;
 
 namespace {
 
@@ -336,6 +340,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown mode"};
     if (opt.task_order != TFSM_ORDER_REFERENCE && opt.task_order != TFSM_ORDER_LEARNED)
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown task order"};
+    if (opt.reserved != 0) throw ArgError{TFSM_ERR_ARGUMENT, "reserved option field must be 0"};
     const bool learned = opt.task_order == TFSM_ORDER_LEARNED;
     if (learned && !c->ranker_installed)
         throw ArgError{TFSM_ERR_ARGUMENT, "learned task order requires an installed ranker"};
@@ -642,16 +647,22 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         va.long_cap = 0;
         va.long_list = nullptr;
         va.long_steps = opt.long_steps == 0 ? kDefaultLongSteps : opt.long_steps;
-        if (opt.long_steps != 0xffffffffu && max_nv <= 8) {
+        if (va.long_steps != 0xffffffffu && max_nv <= 8) {
             va.long_cap = (uint32_t)std::min<unsigned long long>(total_tasks, kLongCap);
             va.long_list = c->work.alloc<uint32_t>(va.long_cap);
         }
+        va.long_donate_every = opt.long_donate_every == 0 ? kDefaultLongDonateEvery : opt.long_donate_every;
+        if (opt.long_donate_every == 0xffffffffu) va.long_donate_every = 0;
+        va.long_fanout = opt.long_fanout == 0 ? kDefaultLongFanout : std::min(opt.long_fanout, 64u);
+        // the second phase's blocks (4 waves each) all fit at once (20 waves per CU)
+        const uint32_t long_blocks_per_cu =
+            std::min(5u, std::max(1u, (opt.long_waves_per_cu ? opt.long_waves_per_cu : kDefaultLongWavesPerCu) / 4));
         if (max_nv <= 8) hipLaunchKernelGGL(k_verify<8>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
         else hipLaunchKernelGGL(k_verify<16>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
         HIPX(hipGetLastError());
         if (va.long_cap) {
-            hipLaunchKernelGGL(k_verify_long<8>, dim3((uint32_t)c->num_cus * kLongBlocksPerCu), dim3(kBlock), 0, s,
-                               va);
+            hipLaunchKernelGGL(k_verify_long<8>, dim3((uint32_t)c->num_cus * long_blocks_per_cu), dim3(kBlock), 0,
+                               s, va, epoch);
             HIPX(hipGetLastError());
         }
         HIPX(hipEventRecord(c->ev[5], s));

@@ -76,24 +76,65 @@ TFSM_WAVE_FN void wave_enter(const A &a, uint32_t u, uint32_t d, WaveStack<MV> &
     s.mask[d] = 0;
 }
 
-// Decides whether u -> candidate ci extends to a full embedding.  On
-// kWaveFound, s.idx holds the embedding (candidate index per pattern vertex).
-// a.abort(u, ci) is polled every `abort_every` rounds (0: never).
-template <int MV, class W, class A>
-TFSM_WAVE_FN uint32_t wave_exists(const W &w, const A &a, uint32_t u, uint32_t ci, WaveStack<MV> &s,
-                                  uint32_t abort_every) {
-    const uint32_t n = a.nv();
+// A search is either a root task (u -> candidate ci, from depth 1) or a
+// piece of one: the candidate indices of the vertices at depths < floor (in
+// depth order), a range [lo, hi) of depth floor's primary row still to test
+// and a chunk of it already tested (positions cpos + i for the set bits i of
+// cmask) still to enter.
+template <int MV, class A>
+TFSM_WAVE_FN void wave_start_root(const A &a, uint32_t u, uint32_t ci, WaveStack<MV> &s) {
     s.idx[u] = ci;
     s.vid[0] = a.vid(u, ci);
+    if (a.nv() > 1) wave_enter<MV>(a, u, 1, s);
+}
+
+template <int MV, class A>
+TFSM_WAVE_FN void wave_start_piece(const A &a, uint32_t u, uint32_t floor, const uint32_t *prefix, uint32_t lo,
+                                   uint32_t hi, uint32_t cpos, unsigned long long cmask, WaveStack<MV> &s) {
+    for (uint32_t j = 0; j < floor; ++j) {
+        const uint32_t x = a.order(u, j);
+        s.idx[x] = prefix[j];
+        s.vid[j] = a.vid(x, prefix[j]);
+    }
+    wave_enter<MV>(a, u, floor, s);
+    s.base[floor] = lo;
+    s.end[floor] = hi;
+    s.chunk[floor] = cpos;
+    s.mask[floor] = cmask;
+}
+
+// Runs a started search until it finds an embedding (kWaveFound: s.idx holds
+// it), exhausts depth `floor` (kWaveExhausted) or h.abort() says stop
+// (kWaveAborted, polled every h.abort_every rounds).  Every h.donate_every
+// rounds it offers the pending work of its shallowest depth k that has some
+// (untested positions [lo, hi) and tested candidates cpos + bits of cmask) to
+// h.donate(k, lo, hi, cpos, cmask, s), keeping work of its own: below k it
+// keeps the subtree it is in, at its own depth it keeps the tested candidates
+// and gives only the untested range.  When h.donate returns true the offered
+// work is gone from this search (the prefix of the piece is s.idx of the
+// vertices at depths < k).  Both hooks return the same value on every lane.
+template <int MV, class W, class A, class H>
+TFSM_WAVE_FN uint32_t wave_run(const W &w, const A &a, uint32_t u, uint32_t floor, WaveStack<MV> &s, H &h) {
+    const uint32_t n = a.nv();
     if (n == 1) return kWaveFound;
-    uint32_t d = 1;
-    wave_enter<MV>(a, u, d, s);
+    uint32_t d = floor;
     for (uint32_t round = 1;; ++round) {
-        if (abort_every && round % abort_every == 0 && a.abort(u, ci)) return kWaveAborted;
+        if (h.abort_every && round % h.abort_every == 0 && h.abort()) return kWaveAborted;
+        if (h.donate_every && round % h.donate_every == 0) {
+            uint32_t k = floor;
+            while (k < d && s.base[k] >= s.end[k] && s.mask[k] == 0) ++k;
+            const unsigned long long gm = k < d ? s.mask[k] : 0ull;
+            const bool gr = s.base[k] < s.end[k] && (k < d || s.mask[k] != 0);
+            if ((gm || gr) && h.donate(k, gr ? s.base[k] : 0u, gr ? s.end[k] : 0u, s.chunk[k], gm, s)) {
+                if (gr) s.end[k] = s.base[k];
+                s.mask[k] &= ~gm;
+            }
+        }
         if (s.mask[d] == 0) {
             const uint32_t b0 = s.base[d], e = s.end[d];
             if (b0 >= e) {
-                if (--d == 0) return kWaveExhausted;
+                if (d == floor) return kWaveExhausted;
+                --d;
                 continue;
             }
             // test the next 64 entries of the primary row, one per lane

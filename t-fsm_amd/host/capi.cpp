@@ -78,9 +78,12 @@ int tfsmh_ctx_create(const tfsmh_graph *h, tfsm_ctx **out) {
 // search_launches, bytes_table, bytes_search, levels, max pattern size, 0.
 // results_path: grami_results.txt format; keys_path: one line per frequent
 // pattern "<dfs key>\t<support lower>\t<support upper>" (either may be NULL).
+// tuning (nullable): search-kernel shape for every evaluation call.
 int64_t tfsmh_mine(const tfsmh_graph *h, tfsm_ctx *ctx, int freq, int max_nodes, int threads, int batched, int mode,
-                   int learned, const char *results_path, const char *keys_path, double *stats_out) {
+                   int learned, const char *results_path, const char *keys_path, double *stats_out,
+                   const tfsm_eval_options *tuning) {
     tfsm::MinerOptions opt;
+    if (tuning) opt.tuning = *tuning;
     opt.freq = freq;
     opt.max_nodes = max_nodes;
     opt.threads = threads;

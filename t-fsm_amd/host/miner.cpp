@@ -99,8 +99,7 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
         for (auto &k : kids)
             for (auto &c : k) level.push_back(Candidate{std::move(c), 0});
     }
-    tfsm_eval_options eo;
-    std::memset(&eo, 0, sizeof(eo));
+    tfsm_eval_options eo = opt.tuning;
     eo.mode = (uint32_t)opt.mode;
     eo.retain_domains = 1;
     eo.task_order = opt.learned ? TFSM_ORDER_LEARNED : TFSM_ORDER_REFERENCE;

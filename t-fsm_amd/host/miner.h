@@ -29,6 +29,9 @@ struct MinerOptions {
     int mode = TFSM_MODE_CAPPED;
     bool learned = false;      // learned task order (requires a ranker on the context)
     bool trace = false;        // keep per-task traces for ranker training
+    // search-kernel shape (grid_waves_per_cu, split_*, long_*); mode, order,
+    // trace and retention are set from the fields above
+    tfsm_eval_options tuning{};
 };
 
 struct FoundPattern {

@@ -45,7 +45,9 @@ class Pattern(ctypes.Structure):
 class EvalOptions(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_uint32), ("retain_domains", ctypes.c_uint32), ("task_order", ctypes.c_uint32),
                 ("trace", ctypes.c_uint32), ("grid_waves_per_cu", ctypes.c_uint32), ("split_steps", ctypes.c_uint32),
-                ("split_fanout", ctypes.c_uint32), ("long_steps", ctypes.c_uint32)]
+                ("split_fanout", ctypes.c_uint32), ("long_steps", ctypes.c_uint32),
+                ("long_waves_per_cu", ctypes.c_uint32), ("long_donate_every", ctypes.c_uint32),
+                ("long_fanout", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class PatternResult(ctypes.Structure):
@@ -163,7 +165,8 @@ def host_lib():
         lib.tfsmh_edge_stats.argtypes = [vp, P(ctypes.c_int32)]
         lib.tfsmh_ctx_create.argtypes = [vp, P(vp)]
         lib.tfsmh_mine.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                   ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, P(ctypes.c_double)]
+                                   ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, P(ctypes.c_double),
+                                   P(EvalOptions)]
         lib.tfsmh_mine.restype = ctypes.c_int64
         lib.tfsmh_is_min_code.argtypes = [P(ctypes.c_int32), ctypes.c_int]
         lib.tfsmh_extend.argtypes = [vp, ctypes.c_int, P(ctypes.c_int32), ctypes.c_int, P(ctypes.c_int32),
@@ -256,27 +259,25 @@ class Context:
             raise TfsmError(rc, "tfsm_ctx_create")
         self.h = h
 
-    def eval_pattern(self, pat, freq, mode=MODE_CAPPED, parent=0, retain=False, trace=False, split_steps=0,
-                     learned=False, split_fanout=0, grid_waves_per_cu=0, long_steps=0):
+    def eval_pattern(self, pat, freq, mode=MODE_CAPPED, parent=0, retain=False, trace=False, learned=False,
+                     **tuning):
         opt = EvalOptions(mode=mode, retain_domains=int(retain),
-                          task_order=ORDER_LEARNED if learned else ORDER_REFERENCE, trace=int(trace),
-                          split_steps=split_steps, split_fanout=split_fanout, grid_waves_per_cu=grid_waves_per_cu,
-                          long_steps=long_steps)
+                          task_order=ORDER_LEARNED if learned else ORDER_REFERENCE, trace=int(trace), **tuning)
         res = PatternResult()
         _check(self.h, self.lib.tfsm_eval_pattern(self.h, ctypes.byref(pat), ctypes.c_uint64(parent), int(freq),
                                                   ctypes.byref(opt), ctypes.byref(res)))
         return res
 
-    def eval_level(self, pats, freq, mode=MODE_CAPPED, parents=None, retain=False, trace=False, split_steps=0,
-                   grid_waves_per_cu=0, learned=False, split_fanout=0, long_steps=0):
+    def eval_level(self, pats, freq, mode=MODE_CAPPED, parents=None, retain=False, trace=False, learned=False,
+                   **tuning):
+        """tuning: grid_waves_per_cu, split_steps, split_fanout, long_steps,
+        long_waves_per_cu, long_donate_every, long_fanout (0: tuned default)."""
         n = len(pats)
         arr = (Pattern * n)(*pats)
         par = (ctypes.c_uint64 * n)(*(parents or [0] * n))
         res = (PatternResult * n)()
         opt = EvalOptions(mode=mode, retain_domains=int(retain),
-                          task_order=ORDER_LEARNED if learned else ORDER_REFERENCE, trace=int(trace),
-                          split_steps=split_steps, grid_waves_per_cu=grid_waves_per_cu, split_fanout=split_fanout,
-                          long_steps=long_steps)
+                          task_order=ORDER_LEARNED if learned else ORDER_REFERENCE, trace=int(trace), **tuning)
         _check(self.h, self.lib.tfsm_eval_level(self.h, arr, par, n, int(freq), ctypes.byref(opt), res))
         return list(res)
 
@@ -386,11 +387,13 @@ class Context:
         _check(self.h, self.lib.tfsm_domain_release(self.h, handle))
 
     def mine(self, freq, max_nodes=-1, threads=1, batched=False, mode=MODE_CAPPED, learned=False,
-             results_path=None, keys_path=None):
+             results_path=None, keys_path=None, **tuning):
+        """tuning: search-kernel shape for every evaluation (see eval_level)."""
         stats = (ctypes.c_double * 15)()
+        opt = EvalOptions(**tuning)
         n = host_lib().tfsmh_mine(self.graph.h, self.h, int(freq), int(max_nodes), int(threads), int(batched),
                                   int(mode), int(learned), results_path.encode() if results_path else None,
-                                  keys_path.encode() if keys_path else None, stats)
+                                  keys_path.encode() if keys_path else None, stats, ctypes.byref(opt))
         if n < 0:
             raise TfsmError(int(n), host_lib().tfsmh_last_error().decode())
         names = ["mining_s", "evaluated", "frequent", "verdicts", "searches", "eval_calls", "filter_ms", "table_ms",

@@ -60,3 +60,26 @@ def test_exact_supports(name, workdir):
         assert mine[k][0] == mine[k][1], k          # exact mode: lower == upper == MNI
         assert mine[k] == theirs[k], k
     ctx.close()
+
+
+@pytest.mark.parametrize("tuning", [dict(long_steps=64, long_donate_every=1, long_fanout=2)], ids=["second-phase"])
+def test_exact_supports_split(tuning, workdir):
+    """Whole batched runs with long searches moved to the wave-cooperative
+    phase after 64 steps and its pieces offered at every round: every level's
+    patterns share one launch, and the exact supports still equal the
+    oracle's."""
+    name = "powerlaw100k_f9000_m5"
+    m = golden_manifest()[name]
+    path = workload_path(m["workload"], m["seed"], workdir)
+    g = tfsm.Graph(path, m["freq"])
+    ctx = tfsm.Context(g)
+    kp = os.path.join(str(workdir), f"{name}_exact_split.keys")
+    ctx.mine(m["freq"], m["max_nodes"], threads=4, batched=True, mode=tfsm.MODE_EXACT, keys_path=kp, **tuning)
+    og = orc.OracleGraph(path, m["freq"])
+    op = os.path.join(str(workdir), f"{name}_oracle_exact_s.keys")
+    og.mine(m["freq"], m["max_nodes"], 8, 1, op)
+    mine, theirs = read_keys(kp), read_keys(op)
+    assert set(mine) == set(theirs) == golden_keys(name)
+    for k in mine:
+        assert mine[k] == theirs[k], k
+    ctx.close()

@@ -202,27 +202,33 @@ def test_split_search_parity(split_steps, fanout, workdir):
         assert bool(cap.frequent) == ref.frequent
 
 
-@pytest.mark.parametrize("long_steps", [64, 256, NEVER])
+@pytest.mark.parametrize("long_opts", [
+    dict(long_steps=64),                                           # second phase, default offers
+    dict(long_steps=64, long_donate_every=1, long_fanout=2),       # offers at every round
+    dict(long_steps=256, long_donate_every=NEVER, long_waves_per_cu=4),
+    dict(long_steps=NEVER),                                        # first phase only
+], ids=["long64", "long64-offer1", "long256-nooffer", "never"])
 @pytest.mark.parametrize("name,seed,freq,golden,limit", [
     ("gnm2k", 1, 40, "gnm2k_f40_m4", 60),
     ("powerlaw100k", 2, 9000, "powerlaw100k_f9000_m5", 40),
 ])
-def test_long_search_parity(long_steps, name, seed, freq, golden, limit, workdir):
+def test_long_search_parity(long_opts, name, seed, freq, golden, limit, workdir):
     """Root searches handed to the wave-cooperative second phase after 64 or
-    256 steps, or never: the oracle's exact per-candidate verdicts and the
-    capped verdicts either way."""
+    256 steps (its pieces offered to idle waves at the default interval, at
+    every round, or never), or never: the oracle's exact per-candidate
+    verdicts and the capped verdicts either way."""
     env = env_for(name, seed, freq, workdir)
     for vl, es in variants(golden_keys(golden), limit):
         pat = env.dense(vl, es)
         if pat is None:
             continue
-        res = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_EXACT, long_steps=long_steps)
+        res = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_EXACT, **long_opts)
         ref = env.og.eval(vl, es, env.freq, mode=1)
         assert (res.support_lower, res.support_upper) == (ref.lower, ref.upper), f"{vl} {es}"
         if ref.decided_by == 2:
             for u in range(len(vl)):
                 assert np.array_equal(env.ctx.last_states(u), ref.states(u)), f"{vl} {es} verdicts {u}"
-        cap = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_CAPPED, long_steps=long_steps)
+        cap = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_CAPPED, **long_opts)
         assert bool(cap.frequent) == ref.frequent
 
 

@@ -5,7 +5,9 @@ with an emulated 64-lane wave over the oracle's filtered domains, auxiliary
 tables and matching orders, with every access bounds-checked (and the harness
 built with the host address and undefined-behaviour sanitizers).  Every
 verdict must equal the oracle's exact-mode verdict, and every embedding found
-must be injective and use only table edges.
+must be injective and use only table edges; with donation on (every round or
+every few rounds, one or several pieces per donation), the verdict combined
+over a task's pieces must be the same.
 """
 import os
 import subprocess
@@ -47,29 +49,37 @@ def variants(keys, limit):
     return out[:limit]
 
 
-def run(harness, path, freq, samples, pats, tmp):
+DONATION = [(0, 1), (1, 1), (3, 4)]     # (donate every N rounds (0: never), pieces per donation)
+
+
+def run(harness, path, freq, samples, pats, tmp, donate=(0, 1)):
     spec = os.path.join(str(tmp), "spec.txt")
     with open(spec, "w") as f:
         f.write(f"{path} {freq} {samples}\n")
         for vl, es in pats:
             f.write(" ".join(str(x) for x in [len(vl)] + list(vl) + [len(es)] + [y for e in es for y in e]) + "\n")
-    r = subprocess.run([harness, spec], capture_output=True, text=True, timeout=600)
+    r = subprocess.run([harness, spec, str(donate[0]), str(donate[1])], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
-    patterns, checked = int(r.stdout.split()[1]), int(r.stdout.split()[3])
+    f = r.stdout.split()
+    patterns, checked, donations = int(f[1]), int(f[3]), int(f[7])
     assert patterns > 0 and checked > 0, r.stdout
-    return patterns, checked
+    assert donate[0] > 0 or donations == 0, r.stdout
+    return patterns, checked, donations
 
 
+@pytest.mark.parametrize("donate", DONATION)
 @pytest.mark.parametrize("golden,freq", [("readme_f1_m3", 1), ("readme_f2_m4", 2)])
-def test_readme(harness, golden, freq, tmp_path):
+def test_readme(harness, golden, freq, donate, tmp_path):
     path = os.path.join(GOLDEN, "readme_sample.lg")
-    run(harness, path, freq, 0, variants(golden_keys(golden), 40), tmp_path)
+    run(harness, path, freq, 0, variants(golden_keys(golden), 40), tmp_path, donate)
 
 
+@pytest.mark.parametrize("donate", DONATION)
 @pytest.mark.parametrize("golden,freq", [("gnm2k_f80_m4", 80), ("gnm2k_f40_m4", 40)])
-def test_gnm(harness, golden, freq, workdir, tmp_path):
+def test_gnm(harness, golden, freq, donate, workdir, tmp_path):
     path = workload_path("gnm2k", 1, workdir)
-    run(harness, path, freq, 0, variants(golden_keys(golden), 200), tmp_path)
+    _, _, donations = run(harness, path, freq, 0, variants(golden_keys(golden), 200), tmp_path, donate)
+    assert (donations > 0) == (donate[0] > 0)
 
 
 def test_powerlaw_sampled(harness, workdir, tmp_path):
@@ -77,4 +87,5 @@ def test_powerlaw_sampled(harness, workdir, tmp_path):
     candidates per vertex (the oracle's exact evaluation bounds the count)."""
     path = workload_path("powerlaw100k", 2, workdir)
     pats = [p for p in variants(golden_keys("powerlaw100k_f9000_m5"), 60) if len(p[0]) >= 4][:14]
-    run(harness, path, 9000, 10, pats, tmp_path)
+    _, _, donations = run(harness, path, 9000, 10, pats, tmp_path, (8, 2))
+    assert donations > 0

@@ -5,8 +5,14 @@
 // of every candidate it is given.  Every memory access the search makes goes
 // through a bounds-checked accessor; a violation exits with status 2.
 //
-//   wave_dfs_check SPEC     (SPEC: see the pattern file written by
-//                            tests/test_wave_dfs.py)
+// Pieces: the search donates the pending work of its shallowest open depth
+// every `donate_every` rounds (the untested range split into up to `fanout`
+// pieces, the tested candidates with the first one); the pieces
+// run one after another, and the task's verdict is "found" when any piece
+// finds an embedding, "exhausted" when every piece is exhausted.
+//
+//   wave_dfs_check SPEC DONATE_EVERY FANOUT   (SPEC: see the pattern file
+//                                             written by tests/test_wave_dfs.py)
 #define TFSM_WAVE_FN inline
 #include "../t-fsm_amd/csrc/wave_dfs.h"
 
@@ -80,16 +86,49 @@ struct HostAcc {
         if (u >= n || d >= n) violation("same_before", u, d);
         return sb[u][d];
     }
-    bool abort(uint32_t, uint32_t) const { return false; }
+};
+
+struct Piece {
+    uint32_t floor, lo, hi, cpos;
+    unsigned long long cmask;
+    uint32_t prefix[16];
+};
+
+struct HostHooks {
+    uint32_t abort_every = 0, donate_every = 0, fanout = 1, u = 0;
+    const HostAcc *a = nullptr;
+    std::vector<Piece> *queue = nullptr;
+    uint64_t donations = 0;
+    bool abort() const { return false; }
+    template <int MV>
+    bool donate(uint32_t k, uint32_t lo, uint32_t hi, uint32_t cpos, unsigned long long cmask,
+                const tfsm_wave::WaveStack<MV> &s) {
+        if (hi < lo || (hi == lo && !cmask) || k == 0 || k >= a->n) violation("donated work", lo, hi);
+        const uint32_t m = std::max<uint32_t>(1, std::min<uint32_t>(fanout, hi - lo));
+        for (uint32_t r = 0; r < m; ++r) {
+            Piece p;
+            p.floor = k;
+            p.lo = lo + (uint32_t)((uint64_t)(hi - lo) * r / m);
+            p.hi = lo + (uint32_t)((uint64_t)(hi - lo) * (r + 1) / m);
+            p.cpos = cpos;
+            p.cmask = r == 0 ? cmask : 0ull;
+            for (uint32_t j = 0; j < k; ++j) p.prefix[j] = s.idx[a->order(u, j)];
+            queue->push_back(p);
+        }
+        ++donations;
+        return true;
+    }
 };
 
 }  // namespace
 
 int main(int argc, char **argv) {
-    if (argc != 2) {
-        std::fprintf(stderr, "usage: wave_dfs_check SPEC\n");
+    if (argc != 4) {
+        std::fprintf(stderr, "usage: wave_dfs_check SPEC DONATE_EVERY FANOUT\n");
         return 64;
     }
+    const uint32_t donate_every = (uint32_t)std::atoi(argv[2]), fanout = (uint32_t)std::atoi(argv[3]);
+    uint64_t donations = 0;
     std::ifstream in(argv[1]);
     std::string graph_path;
     int freq = 0;
@@ -160,18 +199,18 @@ int main(int argc, char **argv) {
             const size_t sz = states.size();
             const size_t step = samples && sz > samples ? sz / samples : 1;
             for (size_t ci = 0; ci < sz; ci += step) {
-                const uint32_t r = tfsm_wave::wave_exists<16>(w, a, (uint32_t)u, (uint32_t)ci, st, 0);
-                const uint8_t want = states[ci];
-                const uint8_t got = r == tfsm_wave::kWaveFound ? 1 : r == tfsm_wave::kWaveExhausted ? 2 : 0;
-                ++checked;
-                if (got != want) {
-                    ++mismatches;
-                    if (mismatches <= 10)
-                        std::fprintf(stderr, "mismatch: pattern %llu u %d ci %zu got %d want %d\n",
-                                     (unsigned long long)patterns, u, ci, got, want);
-                }
-                if (got == 1) {
-                    // the embedding is injective, edge-consistent and maps u to ci
+                std::vector<Piece> queue;
+                HostHooks h;
+                h.donate_every = donate_every;
+                h.fanout = fanout;
+                h.u = (uint32_t)u;
+                h.a = &a;
+                h.queue = &queue;
+                tfsm_wave::wave_start_root<16>(a, (uint32_t)u, (uint32_t)ci, st);
+                uint32_t r = tfsm_wave::wave_run<16>(w, a, (uint32_t)u, 1, st, h);
+                bool found = r == tfsm_wave::kWaveFound, all_exhausted = r == tfsm_wave::kWaveExhausted;
+                auto check_embedding = [&]() {
+                    // injective, edge-consistent and mapping u to ci
                     if (st.idx[u] != ci) violation("embedding root", st.idx[u], ci);
                     for (int i = 0; i < nv; ++i)
                         for (int j = i + 1; j < nv; ++j)
@@ -183,13 +222,35 @@ int main(int argc, char **argv) {
                         for (uint32_t q = o[st.idx[pa]]; q < o[st.idx[pa] + 1]; ++q) hit |= a.pool_[q] == st.idx[pb];
                         if (!hit) violation("embedding edge missing", pa, pb);
                     }
+                };
+                if (found) check_embedding();
+                for (size_t q = 0; q < queue.size(); ++q) {
+                    const Piece p = queue[q];
+                    if (p.prefix[0] != ci) violation("piece root", p.prefix[0], ci);
+                    tfsm_wave::wave_start_piece<16>(a, (uint32_t)u, p.floor, p.prefix, p.lo, p.hi, p.cpos, p.cmask, st);
+                    r = tfsm_wave::wave_run<16>(w, a, (uint32_t)u, p.floor, st, h);
+                    if (r == tfsm_wave::kWaveFound) {
+                        found = true;
+                        check_embedding();
+                    }
+                    all_exhausted &= r == tfsm_wave::kWaveExhausted;
+                }
+                donations += h.donations;
+                const uint8_t want = states[ci];
+                const uint8_t got = found ? 1 : all_exhausted ? 2 : 0;
+                ++checked;
+                if (got != want) {
+                    ++mismatches;
+                    if (mismatches <= 10)
+                        std::fprintf(stderr, "mismatch: pattern %llu u %d ci %zu got %d want %d\n",
+                                     (unsigned long long)patterns, u, ci, got, want);
                 }
             }
         }
         orc_eval_free(e);
     }
     orc_graph_free(g);
-    std::printf("patterns %llu checked %llu mismatches %llu\n", (unsigned long long)patterns,
-                (unsigned long long)checked, (unsigned long long)mismatches);
+    std::printf("patterns %llu checked %llu mismatches %llu donations %llu\n", (unsigned long long)patterns,
+                (unsigned long long)checked, (unsigned long long)mismatches, (unsigned long long)donations);
     return mismatches ? 1 : 0;
 }
