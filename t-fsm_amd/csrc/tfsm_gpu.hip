@@ -614,8 +614,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
             float *logits = c->work.alloc<float>(total_tasks);
             float *sums = c->work.alloc<float>((size_t)n * kMaxV);
             uint32_t *keys = c->work.alloc<uint32_t>(total_tasks);
-            // counting sort: pg blocks own contiguous task chunks (pg <= 128
-            // keeps the one-block scan of pg * 512 counters short)
+            // counting sort: pg blocks own contiguous task chunks (pg <= 128)
             const uint32_t pg = (uint32_t)std::min<unsigned long long>(128, (total_tasks + kBlock - 1) / kBlock);
             const uint32_t chunk = (uint32_t)((total_tasks + pg - 1) / pg);
             uint32_t *hist = c->work.alloc<uint32_t>((size_t)kPrioBins * pg);
@@ -626,7 +625,12 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
                                sums);
             hipLaunchKernelGGL(k_priority, dim3(pg), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, chunk,
                                logits, sums, keys, hist);
-            hipLaunchKernelGGL(k_prio_scan, dim3(1), dim3(kBlock), 0, s, hist, kPrioBins * pg);
+            const uint32_t nhist = kPrioBins * pg, ntiles = (nhist + kScanTile - 1) / kScanTile;
+            static_assert((size_t)kPrioBins * 128 <= (size_t)kScanTile * kBlock, "scan of the sums is one block");
+            uint32_t *tiles = c->work.alloc<uint32_t>(ntiles);
+            hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, hist, nhist, tiles);
+            hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, tiles, ntiles);
+            hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(kBlock), 0, s, hist, nhist, tiles);
             hipLaunchKernelGGL(k_prio_scatter, dim3(pg), dim3(kBlock), 0, s, keys, total_tasks, chunk, hist, order);
             HIPX(hipGetLastError());
             va.order = order;

@@ -5,16 +5,20 @@
 // DFS node spread over the wave: the candidates of depth d are the entries of
 // the shortest backward-neighbour row that occur in every other
 // backward-neighbour row (generateValidCandidateIndex, gmatch.h:1508-1546);
-// the wave tests 64 consecutive entries of that row at once (one lane each:
-// membership in the other rows by binary search, then injectivity against
-// earlier vertices with the same label), keeps the passing ones as a 64-bit
-// mask, and descends into them one at a time in ascending order.  For long
-// searches this divides the sequential chain of dependent loads by up to 64.
+// the wave tests 64 consecutive entries of that row at once (one lane each),
+// keeps the passing ones as a 64-bit mask, and descends into them one at a
+// time in ascending order.  The chunk's candidates are staged in LDS; each
+// other row is walked by a forward cursor in windows of 64 entries staged in
+// LDS, and every lane whose candidate falls inside the window looks it up
+// there (merge of two ascending runs, one global load per window instead of
+// a binary search in global memory per lane).  Injectivity is checked against
+// earlier vertices with the same label.
 //
 // The search state is uniform across the wave and lives in a WaveStack; every
-// lane runs the uniform code with identical values, and only the per-candidate
-// test differs by lane.  The code is generic over the wave (W: ballot over the
-// 64 lanes) and the data accessors (A), so that the same source runs on the
+// lane runs the uniform code with identical values.  Per-lane results travel
+// only as ballots (64-bit masks), and staging writes one entry per lane, so
+// the code is generic over the wave (W: ballot over the 64 lanes, stage into
+// a shared array) and the data accessors (A), and the same source runs on the
 // device and in a host check against the CPU oracle.  The includer defines
 // TFSM_WAVE_FN (the function qualifiers for its side).
 #pragma once
@@ -37,17 +41,20 @@ struct WaveStack {
     uint32_t prim[MV];               // which backward-neighbour row is primary
     unsigned long long mask[MV];     // passing candidates of the chunk not yet entered
     uint32_t rlo[MV][MV], rhi[MV][MV];   // backward-neighbour rows of depth d
+    uint32_t cur[MV][MV];            // forward cursor into each non-primary row of depth d
+    uint32_t cand[kLanes];           // the chunk under test (candidate index per lane)
+    uint32_t win[kLanes];            // a window of a non-primary row
 };
 
 // Index of the lowest set bit (m != 0).
 TFSM_WAVE_FN uint32_t wave_ctz64(unsigned long long m) { return (uint32_t)__builtin_ctzll(m); }
 
-// Is c in the ascending run a.pool(lo .. hi-1)?
-template <class A>
-TFSM_WAVE_FN bool wave_contains(const A &a, uint32_t lo, uint32_t hi, uint32_t c) {
+// Is c among the ascending values w[0 .. n-1]?  (LDS on the device)
+TFSM_WAVE_FN bool wave_in_window(const uint32_t *w, uint32_t n, uint32_t c) {
+    uint32_t lo = 0, hi = n;
     while (lo < hi) {
-        const uint32_t mid = lo + ((hi - lo) >> 1);
-        const uint32_t x = a.pool(mid);
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t x = w[mid];
         if (x == c) return true;
         if (x < c) lo = mid + 1; else hi = mid;
     }
@@ -65,6 +72,7 @@ TFSM_WAVE_FN void wave_enter(const A &a, uint32_t u, uint32_t d, WaveStack<MV> &
         a.row(u, d, k, s.idx[a.bn_vertex(u, d, k)], lo, hi);
         s.rlo[d][k] = lo;
         s.rhi[d][k] = hi;
+        s.cur[d][k] = lo;
         if (hi - lo < best) {
             best = hi - lo;
             prim = k;
@@ -113,6 +121,8 @@ TFSM_WAVE_FN void wave_start_piece(const A &a, uint32_t u, uint32_t floor, const
 // and gives only the untested range.  When h.donate returns true the offered
 // work is gone from this search (the prefix of the piece is s.idx of the
 // vertices at depths < k).  Both hooks return the same value on every lane.
+// h.entered(d, c) is told of every candidate the search enters (a check hook;
+// empty on the device).
 template <int MV, class W, class A, class H>
 TFSM_WAVE_FN uint32_t wave_run(const W &w, const A &a, uint32_t u, uint32_t floor, WaveStack<MV> &s, H &h) {
     const uint32_t n = a.nv();
@@ -137,21 +147,46 @@ TFSM_WAVE_FN uint32_t wave_run(const W &w, const A &a, uint32_t u, uint32_t floo
                 --d;
                 continue;
             }
-            // test the next 64 entries of the primary row, one per lane
+            // test the next (up to) 64 entries of the primary row, one per lane
             const uint32_t ud = a.order(u, d), nb = a.bn_n(u, d), prim = s.prim[d];
+            const uint32_t cn = e - b0 > kLanes ? kLanes : e - b0;
+            w.stage(s.cand, cn, [&](uint32_t i) { return a.pool(b0 + i); });
+            unsigned long long pass = cn == kLanes ? ~0ull : (1ull << cn) - 1ull;
+            for (uint32_t k = 0; k < nb && pass; ++k) {
+                if (k == prim) continue;
+                // merge with row k: windows of 64 from the cursor; a lane is
+                // settled once its candidate is <= the window's last entry
+                unsigned long long open = pass, hit = 0;
+                uint32_t cur = s.cur[d][k];
+                const uint32_t hi = s.rhi[d][k];
+                while (open) {
+                    const uint32_t wn = hi - cur > kLanes ? kLanes : hi - cur;
+                    if (wn == 0) break;                     // row exhausted: the open lanes miss
+                    w.stage(s.win, wn, [&](uint32_t i) { return a.pool(cur + i); });
+                    const uint32_t wmax = s.win[wn - 1];
+                    const unsigned long long settled = open & w.ballot([&](uint32_t lane) -> bool {
+                        return s.cand[lane] <= wmax;
+                    });
+                    hit |= settled & w.ballot([&](uint32_t lane) -> bool {
+                        return ((settled >> lane) & 1ull) && wave_in_window(s.win, wn, s.cand[lane]);
+                    });
+                    open &= ~settled;
+                    if (open) cur += wn;                    // later candidates lie beyond this window
+                }
+                s.cur[d][k] = cur;                          // later chunks start no earlier
+                pass &= hit;
+            }
             const uint32_t before = a.same_before(u, d);
-            s.mask[d] = w.ballot([&](uint32_t lane) -> bool {
-                if (lane >= e - b0) return false;
-                const uint32_t c = a.pool(b0 + lane);
-                for (uint32_t k = 0; k < nb; ++k)
-                    if (k != prim && !wave_contains(a, s.rlo[d][k], s.rhi[d][k], c)) return false;
-                if (before) {
-                    const uint32_t v = a.vid(ud, c);
+            if (before && pass) {
+                pass &= w.ballot([&](uint32_t lane) -> bool {
+                    if (!((pass >> lane) & 1ull)) return false;
+                    const uint32_t v = a.vid(ud, s.cand[lane]);
                     for (uint32_t m = before; m; m &= m - 1)
                         if (s.vid[wave_ctz64(m)] == v) return false;
-                }
-                return true;
-            });
+                    return true;
+                });
+            }
+            s.mask[d] = pass;
             s.chunk[d] = b0;
             s.base[d] = e - b0 > kLanes ? b0 + kLanes : e;
             continue;
@@ -161,7 +196,8 @@ TFSM_WAVE_FN uint32_t wave_run(const W &w, const A &a, uint32_t u, uint32_t floo
         const uint32_t i = wave_ctz64(m);
         s.mask[d] = m & (m - 1);
         const uint32_t ud = a.order(u, d);
-        const uint32_t c = a.pool(s.chunk[d] + i);
+        const uint32_t c = a.pool(s.chunk[d] + i);   // s.cand may hold a deeper depth's chunk by now
+        h.entered(d, c);
         s.idx[ud] = c;
         s.vid[d] = a.vid(ud, c);
         if (d + 1 == n) return kWaveFound;

@@ -89,3 +89,17 @@ def test_powerlaw_sampled(harness, workdir, tmp_path):
     pats = [p for p in variants(golden_keys("powerlaw100k_f9000_m5"), 60) if len(p[0]) >= 4][:14]
     _, _, donations = run(harness, path, 9000, 10, pats, tmp_path, (8, 2))
     assert donations > 0
+
+
+@pytest.mark.parametrize("seed,instances,donate", [(1, 30, (0, 1)), (2, 30, (0, 1)), (3, 8, (3, 4)), (4, 8, (1, 2))])
+def test_random_long_rows(harness, seed, instances, donate):
+    """Synthetic tables with rows far longer than the 64-entry chunks and
+    windows, against a plain recursive search written in the harness: the same
+    verdicts, and without pieces the same candidates entered in the same order
+    (so a window or cursor that skips or repeats a table entry shows up)."""
+    r = subprocess.run([harness, "--random", str(seed), str(instances), str(donate[0]), str(donate[1])],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    f = r.stdout.split()
+    checked, found = int(f[2]), int(f[8])
+    assert checked > 1000 and 0 < found < checked, r.stdout

@@ -13,6 +13,12 @@
 //
 //   wave_dfs_check SPEC DONATE_EVERY FANOUT   (SPEC: see the pattern file
 //                                             written by tests/test_wave_dfs.py)
+//
+// Random mode: synthetic patterns and tables with long rows (well past the
+// 64-entry chunks and windows), checked against a plain recursive search
+// written here independently of the wave code:
+//
+//   wave_dfs_check --random SEED INSTANCES DONATE_EVERY FANOUT
 #define TFSM_WAVE_FN inline
 #include "../t-fsm_amd/csrc/wave_dfs.h"
 
@@ -40,6 +46,14 @@ struct HostWave {
         for (uint32_t l = 0; l < tfsm_wave::kLanes; ++l)
             if (f(l)) m |= 1ull << l;
         return m;
+    }
+    // dst[i] = f(i) for i < count (one lane each on the device); the rest of
+    // dst is poisoned so that a read of an unstaged entry shows up
+    template <class F>
+    void stage(uint32_t *dst, uint32_t count, F f) const {
+        if (count > tfsm_wave::kLanes) violation("stage count", count, tfsm_wave::kLanes);
+        for (uint32_t i = 0; i < count; ++i) dst[i] = f(i);
+        for (uint32_t i = count; i < tfsm_wave::kLanes; ++i) dst[i] = 0xdeadbeefu;
     }
 };
 
@@ -99,7 +113,11 @@ struct HostHooks {
     const HostAcc *a = nullptr;
     std::vector<Piece> *queue = nullptr;
     uint64_t donations = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> *seq = nullptr;   // entered (depth, candidate), when set
     bool abort() const { return false; }
+    void entered(uint32_t d, uint32_t c) {
+        if (seq) seq->emplace_back(d, c);
+    }
     template <int MV>
     bool donate(uint32_t k, uint32_t lo, uint32_t hi, uint32_t cpos, unsigned long long cmask,
                 const tfsm_wave::WaveStack<MV> &s) {
@@ -120,9 +138,168 @@ struct HostHooks {
     }
 };
 
+// Random instances.  Vertices 0..n-1 with labels in {0, 1}; a connected
+// pattern; domains of random size with random data vertex ids (collisions
+// only within a label, as in a real graph); a table for every pattern edge in
+// both directions with random ascending rows, some far longer than 64.
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    uint32_t below(uint32_t n) { return (uint32_t)(next() % n); }
+};
+
+void random_instance(Rng &rng, HostAcc &a, std::vector<std::vector<int>> &adj) {
+    const int n = 3 + (int)rng.below(3);
+    a = HostAcc();
+    a.n = (uint32_t)n;
+    std::vector<int> lab(n);
+    for (auto &l : lab) l = (int)rng.below(2);
+    adj.assign(n, std::vector<int>(n, 0));
+    for (int x = 1; x < n; ++x) {                 // a spanning tree, then extra edges
+        const int y = (int)rng.below((uint32_t)x);
+        adj[x][y] = adj[y][x] = 1;
+    }
+    for (int t = 0; t < n; ++t) {
+        const int x = (int)rng.below(n), y = (int)rng.below(n);
+        if (x != y && rng.below(3) == 0) adj[x][y] = adj[y][x] = 1;
+    }
+    a.dom.resize(n);
+    for (int x = 0; x < n; ++x) {
+        const uint32_t sz = 40 + rng.below(400);
+        std::vector<uint32_t> ids;
+        uint32_t v = (uint32_t)lab[x];
+        for (uint32_t i = 0; i < sz; ++i) {
+            v += 2 * (1 + rng.below(3));          // ascending, parity = label
+            ids.push_back(v);
+        }
+        a.dom[x] = ids;
+    }
+    a.off.assign(n, std::vector<std::vector<uint32_t>>(n));
+    for (int b = 0; b < n; ++b)
+        for (int x = 0; x < n; ++x) {
+            if (!adj[b][x]) continue;
+            std::vector<uint32_t> o{(uint32_t)a.pool_.size()};
+            const uint32_t cx = (uint32_t)a.dom[x].size();
+            for (size_t r = 0; r < a.dom[b].size(); ++r) {
+                const uint32_t kind = rng.below(8);
+                const uint32_t len = kind == 0 ? 0 : kind < 5 ? rng.below(12) : kind < 7 ? 40 + rng.below(120) : cx / 2;
+                std::vector<uint32_t> row;
+                for (uint32_t c = 0; c < cx; ++c)
+                    if (rng.below(cx) < len) row.push_back(c);
+                a.pool_.insert(a.pool_.end(), row.begin(), row.end());
+                o.push_back((uint32_t)a.pool_.size());
+            }
+            a.off[b][x] = std::move(o);
+        }
+    a.ord.assign(n, std::vector<uint32_t>());
+    a.bn.assign(n, std::vector<std::vector<uint32_t>>(n));
+    a.sb.assign(n, std::vector<uint32_t>(n, 0));
+    for (int u = 0; u < n; ++u) {                 // BFS order from u
+        std::vector<int> seen(n, 0);
+        a.ord[u].push_back((uint32_t)u);
+        seen[u] = 1;
+        for (size_t q = 0; q < a.ord[u].size(); ++q)
+            for (int y = 0; y < n; ++y)
+                if (adj[a.ord[u][q]][y] && !seen[y]) {
+                    seen[y] = 1;
+                    a.ord[u].push_back((uint32_t)y);
+                }
+        for (int d = 0; d < n; ++d)
+            for (int j = 0; j < d; ++j) {
+                if (adj[a.ord[u][j]][a.ord[u][d]]) a.bn[u][d].push_back(a.ord[u][j]);
+                if (lab[a.ord[u][j]] == lab[a.ord[u][d]]) a.sb[u][d] |= 1u << j;
+            }
+    }
+}
+
+// Plain recursive existence search over the same tables: candidates of depth
+// d are the entries of the first backward row present in every other one,
+// entered in ascending order (recorded in seq).
+bool plain_exists(const HostAcc &a, uint32_t u, std::vector<uint32_t> &idx, uint32_t d,
+                  std::vector<std::pair<uint32_t, uint32_t>> &seq) {
+    const uint32_t n = a.n;
+    if (d == n) return true;
+    const uint32_t x = a.ord[u][d];
+    uint32_t lo0, hi0;
+    a.row(u, d, 0, idx[a.bn[u][d][0]], lo0, hi0);
+    for (uint32_t p = lo0; p < hi0; ++p) {
+        const uint32_t c = a.pool_[p];
+        bool ok = true;
+        for (uint32_t k = 1; k < a.bn[u][d].size() && ok; ++k) {
+            uint32_t lo, hi;
+            a.row(u, d, k, idx[a.bn[u][d][k]], lo, hi);
+            bool in = false;
+            for (uint32_t q = lo; q < hi; ++q) in |= a.pool_[q] == c;
+            ok = in;
+        }
+        for (uint32_t j = 0; j < d && ok; ++j)
+            if (a.dom[a.ord[u][j]][idx[a.ord[u][j]]] == a.dom[x][c]) ok = false;
+        if (!ok) continue;
+        seq.emplace_back(d, c);
+        idx[x] = c;
+        if (plain_exists(a, u, idx, d + 1, seq)) return true;
+    }
+    return false;
+}
+
+int random_mode(uint64_t seed, uint32_t instances, uint32_t donate_every, uint32_t fanout) {
+    Rng rng{seed};
+    uint64_t checked = 0, mismatches = 0, donations = 0, found_total = 0;
+    tfsm_wave::WaveStack<16> st;
+    HostWave w;
+    for (uint32_t it = 0; it < instances; ++it) {
+        HostAcc a;
+        std::vector<std::vector<int>> adj;
+        random_instance(rng, a, adj);
+        for (uint32_t u = 0; u < a.n; ++u)
+            for (uint32_t ci = 0; ci < a.dom[u].size(); ci += 1 + rng.below(4)) {
+                std::vector<uint32_t> idx(a.n, 0);
+                idx[u] = ci;
+                std::vector<std::pair<uint32_t, uint32_t>> want_seq, got_seq;
+                const bool want = plain_exists(a, u, idx, 1, want_seq);
+                std::vector<Piece> queue;
+                HostHooks h;
+                if (!donate_every) h.seq = &got_seq;      // without pieces the visit order is the plain one
+                h.donate_every = donate_every;
+                h.fanout = fanout;
+                h.u = u;
+                h.a = &a;
+                h.queue = &queue;
+                tfsm_wave::wave_start_root<16>(a, u, ci, st);
+                uint32_t r = tfsm_wave::wave_run<16>(w, a, u, 1, st, h);
+                bool found = r == tfsm_wave::kWaveFound;
+                for (size_t q = 0; q < queue.size(); ++q) {
+                    const Piece p = queue[q];
+                    tfsm_wave::wave_start_piece<16>(a, u, p.floor, p.prefix, p.lo, p.hi, p.cpos, p.cmask, st);
+                    r = tfsm_wave::wave_run<16>(w, a, u, p.floor, st, h);
+                    found |= r == tfsm_wave::kWaveFound;
+                }
+                donations += h.donations;
+                found_total += found;
+                ++checked;
+                const bool bad = found != want || (!donate_every && got_seq != want_seq);
+                if (bad && ++mismatches <= 10)
+                    std::fprintf(stderr, "mismatch: instance %u u %u ci %u got %d want %d (entered %zu vs %zu)\n", it,
+                                 u, ci, found, want, got_seq.size(), want_seq.size());
+            }
+    }
+    std::printf("random checked %llu mismatches %llu donations %llu found %llu\n", (unsigned long long)checked,
+                (unsigned long long)mismatches, (unsigned long long)donations, (unsigned long long)found_total);
+    return mismatches ? 1 : 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
+    if (argc == 6 && std::string(argv[1]) == "--random")
+        return random_mode(std::strtoull(argv[2], nullptr, 10), (uint32_t)std::atoi(argv[3]),
+                           (uint32_t)std::atoi(argv[4]), (uint32_t)std::atoi(argv[5]));
     if (argc != 4) {
         std::fprintf(stderr, "usage: wave_dfs_check SPEC DONATE_EVERY FANOUT\n");
         return 64;
