@@ -96,8 +96,9 @@ TFSM_WAVE_FN void wave_start_root(const A &a, uint32_t u, uint32_t ci, WaveStack
     if (a.nv() > 1) wave_enter<MV>(a, u, 1, s);
 }
 
-template <int MV, class A>
-TFSM_WAVE_FN void wave_start_piece(const A &a, uint32_t u, uint32_t floor, const uint32_t *prefix, uint32_t lo,
+// prefix: anything with prefix[j] (a pointer, or a reader of a record)
+template <int MV, class A, class Prefix>
+TFSM_WAVE_FN void wave_start_piece(const A &a, uint32_t u, uint32_t floor, const Prefix &prefix, uint32_t lo,
                                    uint32_t hi, uint32_t cpos, unsigned long long cmask, WaveStack<MV> &s) {
     for (uint32_t j = 0; j < floor; ++j) {
         const uint32_t x = a.order(u, j);
