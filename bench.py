@@ -28,6 +28,7 @@ sys.path.insert(0, os.path.join(REPO, "t-fsm_amd"))
 WORKLOAD = dict(name="powerlaw100k", seed=2, freq=9000, max_nodes=5, vertices=100000, edges=1100000)
 HBM_PEAK_GBPS = 8000.0
 RANKER = os.path.join(REPO, "t-fsm_amd", "rankers", "powerlaw.f32")
+TUNING = os.path.join(REPO, "t-fsm_amd", "tuning", "powerlaw100k_f9000_m5.json")
 
 
 def host_threads():
@@ -94,20 +95,24 @@ def main():
     graph = tfsm.Graph(lg, WORKLOAD["freq"])
     ctx = tfsm.Context(graph)             # graph now resident in HBM
     ctx.ranker_load(RANKER)
+    # the search-kernel shape chosen by t-fsm_amd/autotune.py for this workload
+    # (library defaults when the file is absent); the same for every path
+    tuning = tfsm.load_tuning(TUNING) if os.path.exists(TUNING) else {}
     gold = golden_keys()
     threads = host_threads()
 
     def run(batched, learned, steps, warmup, tag):
         keys_path = os.path.join(tmp, f"keys_{tag}.txt")
         for _ in range(warmup):
-            ctx.mine(WORKLOAD["freq"], WORKLOAD["max_nodes"], threads=threads, batched=batched, learned=learned)
+            ctx.mine(WORKLOAD["freq"], WORKLOAD["max_nodes"], threads=threads, batched=batched, learned=learned,
+                     **tuning)
         torch.cuda.synchronize()
         agg = dict(evaluated=0, frequent=0, verdicts=0, searches=0, search_ms=0.0, search_launches=0,
                    bytes_search=0, filter_ms=0.0, table_ms=0.0, bytes_table=0)
         t0 = time.perf_counter()
         for i in range(steps):
             n, st = ctx.mine(WORKLOAD["freq"], WORKLOAD["max_nodes"], threads=threads, batched=batched,
-                             learned=learned, keys_path=keys_path if i == 0 else None)
+                             learned=learned, keys_path=keys_path if i == 0 else None, **tuning)
             for k in agg:
                 agg[k] += st[k]
         torch.cuda.synchronize()
@@ -158,7 +163,7 @@ def main():
         "config": {"workload": "powerlaw100k_chunglu_freq9000_maxnodes5", "vertices": WORKLOAD["vertices"],
                    "edges": WORKLOAD["edges"], "freq": WORKLOAD["freq"], "max_nodes": WORKLOAD["max_nodes"],
                    "path": "batched+learned", "ranker": "t-fsm_amd/rankers/powerlaw.f32 (trained on seeds 11, 12)",
-                   "host_threads": threads},
+                   "tuning": tuning, "host_threads": threads},
         "verifications_per_s": round(agg["verdicts"] / learn_s, 1),
         "frequent_patterns_per_s": round(nfreq * K / learn_s, 3),
         "patterns_evaluated_per_step": evaluated,

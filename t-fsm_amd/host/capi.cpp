@@ -9,6 +9,7 @@
 #include "miner.h"
 #include "pattern.h"
 #include "ranker.h"
+#include "tuning.h"
 
 extern "C" {
 
@@ -67,6 +68,16 @@ void tfsmh_edge_stats(const tfsmh_graph *h, int32_t *out) {
 }
 
 int tfsmh_load_ranker(tfsm_ctx *ctx, const char *path) { return tfsm::load_ranker_file(ctx, path); }
+
+// Search-kernel shape from a tuning file into *opt; 0 or -1 (message in tfsmh_last_error).
+int tfsmh_load_tuning(const char *path, tfsm_eval_options *opt) {
+    std::string err;
+    if (!tfsm::load_tuning_file(path, *opt, err)) {
+        g_err = err;
+        return -1;
+    }
+    return 0;
+}
 
 int tfsmh_ctx_create(const tfsmh_graph *h, tfsm_ctx **out) {
     tfsm_csr c = tfsm::csr_view(h->g);

@@ -2,11 +2,12 @@
 //
 //   run -file G.lg -freq F [-thread T] [-maxNodes K]
 //       [-gpu plain|batched] [-support capped|exact] [-learned RANKER]
-//       [-out grami_results.txt] [-stats stats.json]
+//       [-tune TUNING.json] [-out grami_results.txt] [-stats stats.json]
 //
 // Same flags and console summary as the reference (parallel/run.cpp:11-82);
 // flags are read by name in any order (the reference reads argv[2] as the
-// file).  -learned takes a ranker weight file written by the ranker trainer.
+// file).  -learned takes a ranker weight file written by the ranker trainer,
+// -tune a search-kernel shape written by the autotuner.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -17,6 +18,7 @@
 #include "graph.h"
 #include "miner.h"
 #include "ranker.h"
+#include "tuning.h"
 
 namespace {
 
@@ -34,7 +36,7 @@ int main(int argc, char **argv) {
     const char *freq_s = flag(argc, argv, "-freq");
     if (!file || !freq_s) {
         std::cerr << "usage: run -file G.lg -freq F [-thread T] [-maxNodes K] [-gpu plain|batched]"
-                     " [-support capped|exact] [-learned RANKER] [-out PATH] [-stats PATH]\n";
+                     " [-support capped|exact] [-learned RANKER] [-tune TUNING] [-out PATH] [-stats PATH]\n";
         return 2;
     }
     tfsm::MinerOptions opt;
@@ -45,6 +47,13 @@ int main(int argc, char **argv) {
     if (const char *s = flag(argc, argv, "-gpu")) opt.batched = std::strcmp(s, "batched") == 0;
     if (const char *s = flag(argc, argv, "-support")) opt.mode = std::strcmp(s, "exact") == 0 ? TFSM_MODE_EXACT : TFSM_MODE_CAPPED;
     const char *ranker_path = flag(argc, argv, "-learned");
+    if (const char *t = flag(argc, argv, "-tune")) {
+        std::string err;
+        if (!tfsm::load_tuning_file(t, opt.tuning, err)) {
+            std::cerr << "run: " << err << "\n";
+            return 2;
+        }
+    }
     std::string out = flag(argc, argv, "-out") ? flag(argc, argv, "-out") : "./grami_results.txt";
     const char *stats_path = flag(argc, argv, "-stats");
 

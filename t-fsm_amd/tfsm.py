@@ -173,6 +173,7 @@ def host_lib():
                                      ctypes.c_int]
         lib.tfsmh_last_error.restype = ctypes.c_char_p
         lib.tfsmh_load_ranker.argtypes = [vp, ctypes.c_char_p]
+        lib.tfsmh_load_tuning.argtypes = [ctypes.c_char_p, P(EvalOptions)]
         _host = lib
     return _host
 
@@ -410,6 +411,17 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+def load_tuning(path):
+    """Search-kernel options of a tuning file (keyword arguments for mine and
+    eval_level), read by the same loader as `run -tune`."""
+    opt = EvalOptions()
+    if host_lib().tfsmh_load_tuning(path.encode(), ctypes.byref(opt)) != 0:
+        raise TfsmError(-1, host_lib().tfsmh_last_error().decode())
+    names = ["grid_waves_per_cu", "split_steps", "split_fanout", "long_steps", "long_waves_per_cu",
+             "long_donate_every", "long_fanout"]
+    return {k: getattr(opt, k) for k in names if getattr(opt, k)}
 
 
 def make_pattern(vlabels, edges):
