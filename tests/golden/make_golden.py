@@ -37,6 +37,11 @@ CONFIGS = {
     "gnm2k_f40_m4": ("gnm2k", 1, 40, 4, "serial", 1),
     "powerlaw100k_f9000_m5": ("powerlaw100k", 2, 9000, 5, "serial", 1),
     "powerlaw100k_f6000_m6": ("powerlaw100k", 2, 6000, 6, "parallel", 8),
+    # R-MAT scale 20 (16 vertex labels, 4 edge labels): at freq 20000 the
+    # reference finds nothing (no single-edge type reaches 9200 distinct
+    # endpoints per side), so the fixture uses freq 9000
+    "rmat20_f9000_m5": ("rmat20", 4, 9000, 5, "serial", 1),
+    "rmat20_f20000_m5": ("rmat20", 4, 20000, 5, "serial", 1),
 }
 
 
