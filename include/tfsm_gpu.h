@@ -82,6 +82,15 @@ enum {
     TFSM_ORDER_LEARNED = 1      /* ordered by the ranker installed with tfsm_ranker_set */
 };
 
+/* Matching order inside one existence search.  The verdicts do not depend on
+ * it; the work does. */
+enum {
+    TFSM_MATCH_ADAPTIVE = 0,    /* next = the unmatched neighbour whose shortest backward row under the
+                                   partial embedding is shortest (patterns with at most 12 edges;
+                                   larger ones use the reference order) */
+    TFSM_MATCH_REFERENCE = 1    /* GQL order fixed per start vertex (gmatch.h:692-727, 803-828) */
+};
+
 typedef struct {
     uint32_t mode;              /* TFSM_MODE_* */
     uint32_t retain_domains;    /* keep a frequent pattern's child domains on the device */
@@ -100,6 +109,7 @@ typedef struct {
     uint32_t long_donate_every; /* its rounds between offers of work to idle waves
                                    (0: tuned default, 0xffffffff: never) */
     uint32_t long_fanout;       /* records per offer (0: tuned default, at most 64) */
+    uint32_t match_order;       /* TFSM_MATCH_* (0: adaptive) */
     uint32_t reserved;          /* must be 0 */
 } tfsm_eval_options;
 

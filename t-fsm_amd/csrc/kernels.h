@@ -74,6 +74,10 @@ struct PatStatic {
     uint16_t same_before[kMaxV][kMaxV]; // earlier depths whose vertex has d's label
     uint8_t same_after[kMaxV][kMaxV];   // 1 when a later depth has d's label
     uint32_t task_prefix[kMaxV + 1];    // reference task order: u-major prefix of |C(u)|
+    // adaptive matching order (lane_dfs.h): on for this pattern, and the
+    // other pattern vertices with each vertex's label
+    uint32_t adaptive;
+    uint16_t same_lab[kMaxV];
 };
 
 // Device-written state of one pattern.
@@ -100,7 +104,8 @@ struct WorkTable {
 
 // A donated piece of one (u, v) search: the candidates [lo, hi) at depth
 // `floor` of the matching order of u, below the fixed prefix of candidate
-// indices chosen at depths 0 .. floor-1.
+// indices chosen at depths 0 .. floor-1 (adaptive matching order: of the
+// pattern vertices pord[0 .. floor]).
 struct SplitRecord {
     uint32_t seq;                          // the launch epoch once the record is written
     uint32_t p, u, ci, floor, cnt_id, lo, hi;
@@ -108,8 +113,9 @@ struct SplitRecord {
     uint32_t prefix[kMaxV];
     uint32_t cpos;                         // wave-cooperative pieces: tested candidates at
     uint32_t cmask_lo, cmask_hi;           //   positions cpos + i, i a set bit of the mask
-    uint32_t pad[4];
+    uint8_t pord[kMaxV];                   // pattern vertex per depth (adaptive order)
 };
+static_assert(sizeof(SplitRecord) == 128, "one record per 128-byte line");
 
 // Global bookkeeping of one search launch (zeroed before each launch).
 // Each hot counter sits on its own 128-byte line so that polling one does not

@@ -147,6 +147,7 @@ struct Pinned {
 struct tfsm_ctx {
     int device = 0;
     int num_cus = 0;
+    int verify_blocks8 = 1, verify_blocks16 = 1;    // resident search workgroups per CU
     hipStream_t stream = nullptr;
     // graph
     uint32_t n = 0, L = 0, EL = 0, K = 0, W = 0;
@@ -341,6 +342,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
     if (opt.task_order != TFSM_ORDER_REFERENCE && opt.task_order != TFSM_ORDER_LEARNED)
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown task order"};
     if (opt.reserved != 0) throw ArgError{TFSM_ERR_ARGUMENT, "reserved option field must be 0"};
+    if (opt.match_order != TFSM_MATCH_ADAPTIVE && opt.match_order != TFSM_MATCH_REFERENCE)
+        throw ArgError{TFSM_ERR_ARGUMENT, "unknown matching order"};
     const bool learned = opt.task_order == TFSM_ORDER_LEARNED;
     if (learned && !c->ranker_installed)
         throw ArgError{TFSM_ERR_ARGUMENT, "learned task order requires an installed ranker"};
@@ -420,6 +423,18 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         bool dl = true;
         for (uint32_t a = 0; a < P.nv; ++a)
             for (uint32_t b = a + 1; b < P.nv; ++b) if (P.vlab[a] == P.vlab[b]) dl = false;
+        for (uint32_t a = 0; a < P.nv; ++a)
+            for (uint32_t b = 0; b < P.nv; ++b)
+                if (a != b && P.vlab[a] == P.vlab[b]) P.same_lab[a] |= (uint16_t)(1u << b);
+        // adaptive matching order: at most kCutMax edges, and the non-tree
+        // edges (the non-primary rows over all depths) fit the cursor slots
+        // of the search kernel instance (8 for <= 8 vertices, else 16) and
+        // the 4-bit depth metadata
+        {
+            const uint32_t slots = P.nv <= 8 ? 8u : 15u;
+            P.adaptive = opt.match_order == TFSM_MATCH_ADAPTIVE && tp.num_edges <= tfsm_lane::kCutMax &&
+                         tp.num_edges - (P.nv - 1) <= slots;
+        }
         distinct[p] = dl;
         acyclic[p] = tp.num_edges == tp.num_vertices - 1;
     }
@@ -641,10 +656,12 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         HIPX(hipEventRecord(c->ev[4], s));
         uint32_t max_nv = 0;
         for (uint32_t p = 0; p < n; ++p) if (decided[p] == TFSM_DECIDED_SEARCH) max_nv = std::max(max_nv, hs[p].nv);
-        // persistent grid: resident blocks per CU (LDS-bound for the 16-deep variant)
-        // (LDS per block: 51 KB for the 8-deep variant, 100 KB for the 16-deep one)
-        const uint32_t per_cu = opt.grid_waves_per_cu ? std::max(1u, opt.grid_waves_per_cu / (kBlock / 64))
-                                                      : (max_nv <= 8 ? 3u : 1u);
+        // persistent grid: at most the resident blocks per CU (registers and
+        // LDS: 52 KB per block for the 8-deep variant, 104 KB for the 16-deep one)
+        const uint32_t resident = (uint32_t)(max_nv <= 8 ? c->verify_blocks8 : c->verify_blocks16);
+        const uint32_t per_cu = opt.grid_waves_per_cu
+                                    ? std::min(resident, std::max(1u, opt.grid_waves_per_cu / (kBlock / 64)))
+                                    : resident;
         const uint32_t grid = (uint32_t)c->num_cus * per_cu;
         // root tasks still running after long_steps steps finish in the
         // wave-cooperative search (patterns of up to 8 vertices)
@@ -808,6 +825,12 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         hipDeviceProp_t prop;
         HIPX(hipGetDeviceProperties(&prop, c->device));
         c->num_cus = prop.multiProcessorCount;
+        // the persistent search grid is sized to what stays resident
+        // (registers and LDS of each instance)
+        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->verify_blocks8, k_verify<8>, kBlock, 0));
+        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->verify_blocks16, k_verify<16>, kBlock, 0));
+        c->verify_blocks8 = std::max(1, c->verify_blocks8);
+        c->verify_blocks16 = std::max(1, c->verify_blocks16);
         HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto &e : c->ev) HIPX(hipEventCreate(&e));
         c->n = g->num_vertices;
@@ -1001,7 +1024,13 @@ int64_t tfsm_last_aux(tfsm_ctx *ctx, uint32_t a, uint32_t b, uint32_t *off, uint
     const uint32_t rows = P.dom_size[a];
     const uint32_t total = ctx->last_state[0].aux_total[slot];
     int rc = guarded(ctx, [&]() -> int {
-        if (off) HIPX(hipMemcpy(off, P.aux_off[slot], std::min(rows + 1, off_cap) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (off) {
+            // the device keeps absolute pool positions; callers get offsets within the table
+            const uint32_t cnt = std::min(rows + 1, off_cap);
+            HIPX(hipMemcpy(off, P.aux_off[slot], cnt * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            const uint32_t base = ctx->last_state[0].aux_base[slot];
+            for (uint32_t i = 0; i < cnt; ++i) off[i] -= base;
+        }
         if (idx && total)
             HIPX(hipMemcpy(idx, ctx->d_pool + ctx->last_state[0].aux_base[slot], std::min(total, idx_cap) * sizeof(uint32_t),
                            hipMemcpyDeviceToHost));

@@ -17,6 +17,8 @@ MODE_CAPPED = 0
 MODE_EXACT = 1
 ORDER_REFERENCE = 0
 ORDER_LEARNED = 1
+MATCH_ADAPTIVE = 0
+MATCH_REFERENCE = 1
 DECIDED_FILTER, DECIDED_EXPRESS, DECIDED_SEARCH = 0, 1, 2
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "no usable GPU",
@@ -47,7 +49,8 @@ class EvalOptions(ctypes.Structure):
                 ("trace", ctypes.c_uint32), ("grid_waves_per_cu", ctypes.c_uint32), ("split_steps", ctypes.c_uint32),
                 ("split_fanout", ctypes.c_uint32), ("long_steps", ctypes.c_uint32),
                 ("long_waves_per_cu", ctypes.c_uint32), ("long_donate_every", ctypes.c_uint32),
-                ("long_fanout", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("long_fanout", ctypes.c_uint32), ("match_order", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class PatternResult(ctypes.Structure):
@@ -272,7 +275,8 @@ class Context:
     def eval_level(self, pats, freq, mode=MODE_CAPPED, parents=None, retain=False, trace=False, learned=False,
                    **tuning):
         """tuning: grid_waves_per_cu, split_steps, split_fanout, long_steps,
-        long_waves_per_cu, long_donate_every, long_fanout (0: tuned default)."""
+        long_waves_per_cu, long_donate_every, long_fanout (0: tuned default),
+        match_order (MATCH_ADAPTIVE or MATCH_REFERENCE)."""
         n = len(pats)
         arr = (Pattern * n)(*pats)
         par = (ctypes.c_uint64 * n)(*(parents or [0] * n))
@@ -420,7 +424,7 @@ def load_tuning(path):
     if host_lib().tfsmh_load_tuning(path.encode(), ctypes.byref(opt)) != 0:
         raise TfsmError(-1, host_lib().tfsmh_last_error().decode())
     names = ["grid_waves_per_cu", "split_steps", "split_fanout", "long_steps", "long_waves_per_cu",
-             "long_donate_every", "long_fanout"]
+             "long_donate_every", "long_fanout", "match_order"]
     return {k: getattr(opt, k) for k in names if getattr(opt, k)}
 
 

@@ -83,3 +83,27 @@ def test_exact_supports_split(tuning, workdir):
     for k in mine:
         assert mine[k] == theirs[k], k
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["rmat20_f20000_m5", "rmat20_f9000_m5"])
+def test_rmat20_pattern_sets(name, workdir):
+    """R-MAT scale 20 (1M vertices, 16.8M edges, 16 vertex and 4 edge labels):
+    plain and batched runs on one loaded graph against the reference's sets.
+    At freq 9000 seven single edges are frequent and every size-3 candidate
+    is evaluated on the GPU and rejected; at freq 20000 nothing is frequent."""
+    m = golden_manifest()[name]
+    path = workload_path(m["workload"], m["seed"], workdir)
+    print(f"{name}: loading", flush=True)
+    g = tfsm.Graph(path, m["freq"])
+    ctx = tfsm.Context(g)
+    for batched in (False, True):
+        print(f"{name}: mining batched={batched}", flush=True)
+        kp = os.path.join(str(workdir), f"{name}_{int(batched)}.keys")
+        n, stats = ctx.mine(m["freq"], m["max_nodes"], threads=4, batched=batched, mode=tfsm.MODE_CAPPED,
+                            keys_path=kp)
+        assert n == m["reported_total"]
+        assert set(read_keys(kp)) == golden_keys(name)
+        print(f"{name}: batched={batched} evaluated={stats['evaluated']} mining_s={stats['mining_s']:.3f}",
+              flush=True)
+    ctx.close()
+    g.close()

@@ -72,11 +72,11 @@ def variants(keys, limit):
     return out[:limit]
 
 
-def check_exact(env, vl, es):
+def check_exact(env, vl, es, **opts):
     pat = env.dense(vl, es)
     if pat is None:
         return None
-    res = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_EXACT)
+    res = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_EXACT, **opts)
     ref = env.og.eval(vl, es, env.freq, mode=1)
     where = f"pattern {vl} {es}"
     assert res.decided_by == ref.decided_by, where
@@ -101,14 +101,19 @@ def check_exact(env, vl, es):
     return res
 
 
+MATCH = [tfsm.MATCH_ADAPTIVE, tfsm.MATCH_REFERENCE]
+MATCH_IDS = ["adaptive", "reference"]
+
+
+@pytest.mark.parametrize("match", MATCH, ids=MATCH_IDS)
 @pytest.mark.parametrize("name,seed,freq,golden", CASES)
-def test_exact_parity(name, seed, freq, golden, workdir):
+def test_exact_parity(name, seed, freq, golden, match, workdir):
     env = env_for(name, seed, freq, workdir)
     cases = variants(golden_keys(golden), 60)
     assert cases
     kinds = set()
     for vl, es in cases:
-        r = check_exact(env, vl, es)
+        r = check_exact(env, vl, es, match_order=match)
         if r is not None:
             kinds.add(r.decided_by)
     assert kinds  # at least one pattern was evaluated
@@ -176,18 +181,20 @@ def test_level_equals_single(workdir):
 NEVER = 0xffffffff
 
 
+@pytest.mark.parametrize("match", MATCH, ids=MATCH_IDS)
 @pytest.mark.parametrize("split_steps,fanout", [
     (64, 1),           # record donation every 64 steps
     (64, 16),          # each donation cut into 16 records
     (NEVER, 1),        # no splitting at all
     (0, 0),            # tuned defaults
 ])
-def test_split_search_parity(split_steps, fanout, workdir):
+def test_split_search_parity(split_steps, fanout, match, workdir):
     """Sub-tree donation through the record queue at a tiny step budget, cut
     into one or many records, and no splitting at all, must all give the
-    oracle's exact per-candidate verdicts."""
+    oracle's exact per-candidate verdicts, in either matching order (a record
+    of the adaptive order carries the pattern vertex of each depth)."""
     env = env_for("powerlaw100k", 2, 9000, workdir)
-    opts = dict(split_steps=split_steps, split_fanout=fanout)
+    opts = dict(split_steps=split_steps, split_fanout=fanout, match_order=match)
     for vl, es in variants(golden_keys("powerlaw100k_f9000_m5"), 15):
         pat = env.dense(vl, es)
         if pat is None:
@@ -200,6 +207,31 @@ def test_split_search_parity(split_steps, fanout, workdir):
                 assert np.array_equal(env.ctx.last_states(u), ref.states(u)), f"{vl} {es} verdicts {u}"
         cap = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_CAPPED, **opts)
         assert bool(cap.frequent) == ref.frequent
+
+
+def test_adaptive_order_less_work(workdir):
+    """The adaptive matching order gives the reference order's exact verdicts
+    with fewer DFS steps in total (traced per task) on the power-law graph's
+    four- and five-vertex patterns."""
+    env = env_for("powerlaw100k", 2, 9000, workdir)
+    steps = {tfsm.MATCH_ADAPTIVE: 0, tfsm.MATCH_REFERENCE: 0}
+    for vl, es in variants(golden_keys("powerlaw100k_f9000_m5"), 40):
+        pat = env.dense(vl, es)
+        if pat is None or len(vl) < 4:
+            continue
+        states = {}
+        for match in steps:
+            res = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_EXACT, trace=True, match_order=match)
+            if res.decided_by != tfsm.DECIDED_SEARCH:
+                break
+            tr = env.ctx.read_trace()
+            steps[match] += int((tr & ((1 << 30) - 1)).sum())
+            states[match] = [env.ctx.last_states(u) for u in range(len(vl))]
+        if len(states) == 2:
+            for a, b in zip(states[tfsm.MATCH_ADAPTIVE], states[tfsm.MATCH_REFERENCE]):
+                assert np.array_equal(a, b), f"{vl} {es}"
+    print("traced steps", steps)
+    assert 0 < steps[tfsm.MATCH_ADAPTIVE] < steps[tfsm.MATCH_REFERENCE]
 
 
 @pytest.mark.parametrize("long_opts", [

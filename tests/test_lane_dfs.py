@@ -1,0 +1,94 @@
+"""CPU check of the lane-per-task existence search (t-fsm_amd/csrc/lane_dfs.h).
+
+The search kernel and this check compile the same search source; here it
+runs on the CPU over the oracle's filtered domains, auxiliary tables and
+matching orders, with every access bounds-checked (and the harness built with
+the host address and undefined-behaviour sanitizers).  In the reference
+matching order and in the adaptive one, every verdict must equal the oracle's
+exact-mode verdict and every embedding found must be injective and use only
+table edges; with pieces donated (every step or every few steps, one or
+several pieces per donation), the verdict combined over a task's pieces must
+be the same.
+"""
+import os
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN, ORACLE, REPO, golden_keys, workload_path
+from test_wave_dfs import variants
+
+HARNESS_SRC = os.path.join(REPO, "tests", "lane_dfs_check.cpp")
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    lib = os.path.join(ORACLE, "_build", "liboracle.so")
+    if not os.path.exists(lib):
+        subprocess.run(["make", "-C", ORACLE], check=True, stdout=subprocess.DEVNULL)
+    out = str(tmp_path_factory.mktemp("lane") / "lane_dfs_check")
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-Wno-unknown-pragmas", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=all", "-o", out, HARNESS_SRC, lib,
+                    f"-Wl,-rpath,{os.path.dirname(lib)}"], check=True)
+    return out
+
+
+DONATION = [(0, 1), (1, 1), (2, 3)]     # (donate every N steps (0: never), pieces per donation)
+
+
+def run(harness, path, freq, samples, pats, tmp, donate=(0, 1)):
+    spec = os.path.join(str(tmp), "spec.txt")
+    with open(spec, "w") as f:
+        f.write(f"{path} {freq} {samples}\n")
+        for vl, es in pats:
+            f.write(" ".join(str(x) for x in [len(vl)] + list(vl) + [len(es)] + [y for e in es for y in e]) + "\n")
+    r = subprocess.run([harness, spec, str(donate[0]), str(donate[1])], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    f = r.stdout.split()
+    out = dict(zip(f[0::2], (int(x) for x in f[1::2])))
+    assert out["patterns"] > 0 and out["checked"] > 0, r.stdout
+    assert donate[0] > 0 or out["pieces"] == 0, r.stdout
+    return out
+
+
+@pytest.mark.parametrize("donate", DONATION)
+@pytest.mark.parametrize("golden,freq", [("readme_f1_m3", 1), ("readme_f2_m4", 2)])
+def test_readme(harness, golden, freq, donate, tmp_path):
+    path = os.path.join(GOLDEN, "readme_sample.lg")
+    run(harness, path, freq, 0, variants(golden_keys(golden), 40), tmp_path, donate)
+
+
+@pytest.mark.parametrize("donate", DONATION)
+@pytest.mark.parametrize("golden,freq", [("gnm2k_f80_m4", 80), ("gnm2k_f40_m4", 40)])
+def test_gnm(harness, golden, freq, donate, workdir, tmp_path):
+    path = workload_path("gnm2k", 1, workdir)
+    out = run(harness, path, freq, 0, variants(golden_keys(golden), 200), tmp_path, donate)
+    assert (out["pieces"] > 0) == (donate[0] > 0), out
+
+
+@pytest.mark.parametrize("donate", [(0, 1), (16, 2)])
+def test_powerlaw_sampled(harness, workdir, tmp_path, donate):
+    """Four- and five-vertex patterns of the power-law graph, 40 sampled
+    candidates per vertex; the adaptive order takes far fewer steps."""
+    path = workload_path("powerlaw100k", 2, workdir)
+    pats = [p for p in variants(golden_keys("powerlaw100k_f9000_m5"), 60) if len(p[0]) >= 4][:14]
+    out = run(harness, path, 9000, 40, pats, tmp_path, donate)
+    assert out["steps_adaptive"] < out["steps_reference"], out
+    assert (out["pieces"] > 0) == (donate[0] > 0), out
+
+
+@pytest.mark.parametrize("seed,instances,donate", [(1, 30, (0, 1)), (2, 30, (0, 1)), (3, 6, (16, 4)),
+                                                   (4, 4, (4, 2))])
+def test_random(harness, seed, instances, donate):
+    """Synthetic tables with empty, short and long rows, against a plain
+    recursive search written in the harness: the same verdicts in both orders,
+    and in the reference order without pieces the same candidates entered in
+    the same order (so a cursor that skips or repeats a table entry shows
+    up)."""
+    r = subprocess.run([harness, "--random", str(seed), str(instances), str(donate[0]), str(donate[1])],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    f = r.stdout.split()
+    out = dict(zip(f[1::2], (int(x) for x in f[2::2])))
+    assert out["checked"] > 1000 and 0 < out["found"] < out["checked"], r.stdout
+    assert (out["pieces"] > 0) == (donate[0] > 0), r.stdout
