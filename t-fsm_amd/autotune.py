@@ -25,12 +25,12 @@ sys.path.insert(0, HERE)
 import tfsm  # noqa: E402
 
 NEVER = 0xffffffff
-# Shapes the tuner may choose from: workgroups of 4 waves, 2 or 3 resident per
-# CU (the search kernel's LDS allows 3), and record splitting after 1024 or
-# 2048 steps with one record per donation, or none.
+# Shapes the tuner may choose from: workgroups of 4 waves, 1 or 2 resident per
+# CU (the library clamps to what stays resident), and record splitting after
+# 64 to 1024 steps with one record per donation, or none.
 SPACE = {
-    "grid_waves_per_cu": [8, 12],
-    "split_steps": [1024, 2048, NEVER],
+    "grid_waves_per_cu": [4, 8],
+    "split_steps": [64, 128, 256, 1024, NEVER],
 }
 
 

@@ -30,6 +30,7 @@ constexpr int kMaxV = TFSM_MAX_PATTERN_VERTICES;
 constexpr int kMaxE = TFSM_MAX_PATTERN_EDGES;
 constexpr int kMaxSlots = 2 * kMaxE;
 constexpr int kBlock = 256;            // threads per block of every kernel
+constexpr int kCutEdges = 12;          // pattern edges of the adaptive matching order
 
 struct DevGraph {
     uint32_t n;        // vertices
@@ -78,6 +79,8 @@ struct PatStatic {
     // other pattern vertices with each vertex's label
     uint32_t adaptive;
     uint16_t same_lab[kMaxV];
+    uint32_t edge_pack[kCutEdges / 4];  // edges a | b << 4, one byte each, 0xff past the last
+    uint32_t cut_ao[2 * kCutEdges];     // slot's row offsets at aux_region + cut_ao[slot]
 };
 
 // Device-written state of one pattern.
@@ -85,7 +88,7 @@ struct PatState {
     uint32_t dom_size[kMaxV];
     uint32_t valid_cnt[kMaxV];
     uint32_t invalid_cnt[kMaxV];
-    uint32_t ndone;
+    uint32_t ndone;                     // vertices that reached freq valid candidates
     uint32_t status;                    // 0 undecided, 1 frequent, 2 infrequent
     unsigned long long searches;
     uint32_t aux_base[kMaxSlots];       // table start in the pool
@@ -148,8 +151,12 @@ struct VerifyArgs {
     unsigned long long total;
     SearchCounters *ctr;
     const uint32_t *pool;
+    const uint32_t *aux_region;            // every slot's row offsets of the call
     uint32_t *trace;                       // nullable, indexed by reference task order
     const uint32_t *order;                 // nullable: search order -> reference task index
+    const uint32_t *order_pu;              // with order: (pattern << 4) | pattern vertex of each entry
+    const uint32_t *split;                 // with order: entries before *split form the first part
+    uint32_t part;                         // 0: every entry, 1: the first part, 2: the rest
     uint32_t abort_stopped;                // capped mode: abandon tasks of a u that has freq valid
     uint32_t steps_per_round;
     uint32_t split_steps;                  // donate work after this many steps (0: never)

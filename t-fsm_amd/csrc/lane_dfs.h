@@ -24,11 +24,17 @@
 //
 // The search state is per lane: columns (C) indexed by pattern vertex (idx,
 // and vid in the adaptive order) or by depth (pos/end of the primary row,
-// vid in the reference order, meta), and cursor slots (cur/cend) handed out
-// to the non-primary rows depth after depth.  The code is generic over the
-// columns (LDS on the device) and the data accessors (A), so the same source
-// runs in the search kernel and in a host check against the CPU oracle.  The
-// includer defines TFSM_LANE_FN (the function qualifiers for its side).
+// vid in the reference order, meta), and cursor slots (cur/cend/curv) handed
+// out to the non-primary rows depth after depth.  Each step is one chain of
+// dependent loads, kept short: a cursor slot caches the entry under its
+// cursor (curv), so a candidate below it is rejected without a load; the
+// next primary entry is prefetched during the current step; the vertex id
+// for the injectivity test is loaded beside the membership test; entering a
+// depth loads the row offsets of every cut edge in one round.  The code is
+// generic over the columns (LDS on the device) and the data accessors (A),
+// so the same source runs in the search kernel and in a host check against
+// the CPU oracle.  The includer defines TFSM_LANE_FN (the function
+// qualifiers for its side).
 #pragma once
 
 #include <cstdint>
@@ -39,6 +45,7 @@ namespace tfsm_lane {
 // non-tree edges than cursor slots, is searched in the reference order).
 constexpr uint32_t kCutMax = 12;
 constexpr uint32_t kNone = 0xffu;
+constexpr uint32_t kUnknown = 0xffffffffu;   // curv: entry under the cursor not loaded yet
 
 enum LaneStep : uint32_t { kLaneRunning = 0, kLaneFound = 1, kLaneExhausted = 2 };
 
@@ -55,34 +62,64 @@ TFSM_LANE_FN uint32_t meta_secn(uint32_t m) { return (m >> 12) & 15u; }
 
 TFSM_LANE_FN uint32_t lane_ctz(uint32_t m) { return (uint32_t)__builtin_ctz(m); }
 
-// First position in [lo, hi) whose entry is >= c (galloping from lo).
+// First position in [lo, hi) whose entry is >= c (galloping from lo); `val`
+// gets that entry (kUnknown at hi), known from the probes without a reload.
 template <class A>
-TFSM_LANE_FN uint32_t lane_gallop(const A &a, uint32_t lo, uint32_t hi, uint32_t c) {
-    if (lo >= hi || a.pool(lo) >= c) return lo;
-    uint32_t step = 1, prev = lo;
-    while (lo + step < hi && a.pool(lo + step) < c) {
+TFSM_LANE_FN uint32_t lane_gallop(const A &a, uint32_t lo, uint32_t hi, uint32_t c, uint32_t &val) {
+    val = kUnknown;
+    if (lo >= hi) return lo;
+    uint32_t v = a.pool(lo);
+    if (v >= c) {
+        val = v;
+        return lo;
+    }
+    uint32_t step = 1, prev = lo, yv = kUnknown;
+    while (lo + step < hi) {
+        v = a.pool(lo + step);
+        if (v >= c) {
+            yv = v;
+            break;
+        }
         prev = lo + step;
         step <<= 1;
     }
     uint32_t x = prev + 1, y = lo + step < hi ? lo + step : hi;
     while (x < y) {
         const uint32_t mid = (x + y) >> 1;
-        if (a.pool(mid) < c) x = mid + 1; else y = mid;
+        const uint32_t m = a.pool(mid);
+        if (m < c) {
+            x = mid + 1;
+        } else {
+            y = mid;
+            yv = m;
+        }
     }
+    if (x < hi) val = yv;
     return x;
 }
 
 // A: nv(), adaptive(), pool(pos), vid(x, c), row(slot, r, lo, hi) (absolute
 // bounds of row r of a slot's table), reference plan order(u, d), bn_n(u, d),
 // bn_slot(u, d, k), slot_src(s), sec_base(u, d), same_before(u, d),
-// same_after(u, d); adaptive order: ne(), edge_x(e), edge_y(e), slot_xy(e)
-// (rows over C(x)), slot_yx(e), same_lab(x) (other vertices with x's label).
-// C: idx(x), vid(j), pos(d), end(d), cur(g), cend(g), meta(d), all lvalues.
+// same_after(u, d); adaptive order: edge_pack(i) (pattern edges, one byte
+// each, x | y << 4, four to a word, 0xff past the last), edge_row(e, from_x,
+// r, lo, hi) (row r of edge e's table from x, over C(x), or from y),
+// same_lab(x) (other vertices with x's label).
+// C: idx(x), vid(j), pos(d), end(d), cur(g), cend(g), curv(g), meta(d), all
+// lvalues.
 template <int MV>
 struct Lane {
     uint32_t u = 0, n = 0, d = 0, floor = 1;
     uint32_t vis = 0;              // adaptive order: pattern vertices of depths 0 .. d
     bool adaptive = false;
+    uint32_t pf_pos = kUnknown, pf_val = 0;   // prefetched pool entry (the pool is read-only)
+    uint32_t ep[kCutMax / 4] = {};            // adaptive order: the pattern's packed edges
+
+    template <class A>
+    TFSM_LANE_FN void load_edges(const A &a) {
+#pragma unroll
+        for (uint32_t i = 0; i < kCutMax / 4; ++i) ep[i] = a.edge_pack(i);
+    }
 
     // Reference order: rows of depth dd, read once with their loads in
     // flight together; primary = `prim`, or the shortest when prim is kNone.
@@ -119,6 +156,7 @@ struct Lane {
             if (g < MV) {
                 c.cur(g) = rl[k];
                 c.cend(g) = rh[k];
+                c.curv(g) = kUnknown;
             }
             ++g;
         }
@@ -128,40 +166,47 @@ struct Lane {
     // Adaptive order: depth dd gets vertex w (chosen here when kNone: the
     // one with the shortest cut row) with primary row `prim` (the shortest
     // of w's rows when kNone).  w's backward rows are numbered in pattern
-    // edge order, which a piece's receiver repeats.
+    // edge order, which a piece's receiver repeats.  The edges come from
+    // registers; every cut edge's row offsets are loaded in one round.
     template <class C, class A>
     TFSM_LANE_FN void enter_adaptive(C &c, const A &a, uint32_t dd, uint32_t w, uint32_t prim) {
-        const uint32_t ne = a.ne();
-        uint32_t lo[kCutMax], hi[kCutMax], to[kCutMax];
+        uint32_t lo[kCutMax], ln[kCutMax];
+        uint32_t cut = 0, dlo = 0, dhi = 0;     // cut edges; their unmatched ends, 4 bits each
 #pragma unroll
         for (uint32_t e = 0; e < kCutMax; ++e) {
-            lo[e] = hi[e] = 0;
-            to[e] = kNone;
-            if (e < ne) {
-                const uint32_t x = a.edge_x(e), y = a.edge_y(e);
-                const bool mx = (vis >> x) & 1u, my = (vis >> y) & 1u;
-                const uint32_t dst = mx ? y : x;
-                if (mx != my && (w == kNone || dst == w)) {
-                    a.row(mx ? a.slot_xy(e) : a.slot_yx(e), c.idx(mx ? x : y), lo[e], hi[e]);
-                    to[e] = dst;
-                }
+            lo[e] = 0;
+            ln[e] = 0xffffffffu;
+            const uint32_t xy = (ep[e >> 2] >> ((e & 3u) * 8u)) & 0xffu;
+            if (xy == 0xffu) continue;
+            const uint32_t x = xy & 15u, y = xy >> 4;
+            const uint32_t mx = (vis >> x) & 1u, my = (vis >> y) & 1u;
+            const uint32_t dst = mx ? y : x;
+            if (mx != my && (w == kNone || dst == w)) {
+                uint32_t l, h;
+                a.edge_row(e, mx != 0, c.idx(mx ? x : y), l, h);
+                lo[e] = l;
+                ln[e] = h - l;
+                cut |= 1u << e;
+                if (e < 8) dlo |= dst << (4u * e);
+                else dhi |= dst << (4u * (e - 8));
             }
         }
+        auto dst_of = [&](uint32_t e) { return e < 8 ? (dlo >> (4u * e)) & 15u : (dhi >> (4u * (e - 8))) & 15u; };
         if (w == kNone) {
             uint32_t best = 0xffffffffu;
 #pragma unroll
             for (uint32_t e = 0; e < kCutMax; ++e)
-                if (to[e] != kNone && hi[e] - lo[e] < best) {
-                    best = hi[e] - lo[e];
-                    w = to[e];
+                if (((cut >> e) & 1u) && ln[e] < best) {
+                    best = ln[e];
+                    w = dst_of(e);
                 }
         }
         uint32_t nb = 0, pick = 0, plen = 0xffffffffu;
 #pragma unroll
         for (uint32_t e = 0; e < kCutMax; ++e)
-            if (to[e] == w && w != kNone) {
-                if (hi[e] - lo[e] < plen) {
-                    plen = hi[e] - lo[e];
+            if (((cut >> e) & 1u) && dst_of(e) == w) {
+                if (ln[e] < plen) {
+                    plen = ln[e];
                     pick = nb;
                 }
                 ++nb;
@@ -169,23 +214,25 @@ struct Lane {
         if (prim == kNone) prim = pick;
         const uint32_t pm = dd ? c.meta(dd - 1) : 0u;
         const uint32_t secb = dd ? meta_secb(pm) + meta_secn(pm) : 0u;
-        uint32_t k = 0, g = secb;
-        c.pos(dd) = c.end(dd) = 0;           // no rows: the depth is exhausted at once
+        uint32_t k = 0, g = secb, plo = 0, phi = 0;
 #pragma unroll
         for (uint32_t e = 0; e < kCutMax; ++e)
-            if (to[e] == w && w != kNone) {
+            if (((cut >> e) & 1u) && dst_of(e) == w) {
                 if (k == prim) {
-                    c.pos(dd) = lo[e];
-                    c.end(dd) = hi[e];
+                    plo = lo[e];
+                    phi = lo[e] + ln[e];
                 } else {
                     if (g < MV) {
                         c.cur(g) = lo[e];
-                        c.cend(g) = hi[e];
+                        c.cend(g) = lo[e] + ln[e];
+                        c.curv(g) = kUnknown;
                     }
                     ++g;
                 }
                 ++k;
             }
+        c.pos(dd) = plo;                     // no rows: the depth is exhausted at once
+        c.end(dd) = phi;
         c.meta(dd) = meta_pack(prim, w, secb, nb ? nb - 1 : 0);
         if (w != kNone) vis |= 1u << w;
     }
@@ -199,6 +246,7 @@ struct Lane {
         d = 1;
         c.idx(u) = ci;
         if (adaptive) {
+            load_edges(a);
             vis = 1u << u;
             c.meta(0) = meta_pack(0, u, 0, 0);
             if (a.same_lab(u)) c.vid(u) = a.vid(u, ci);
@@ -221,6 +269,7 @@ struct Lane {
         floor = floor_;
         d = floor_;
         if (adaptive) {
+            load_edges(a);
             vis = 0;
             for (uint32_t j = 0; j < floor_; ++j) {
                 const uint32_t w = pord[j] & 15u;
@@ -260,56 +309,65 @@ struct Lane {
     // backtrack one depth.
     template <class C, class A>
     TFSM_LANE_FN uint32_t step(C &c, const A &a) {
-        const uint32_t pos = c.pos(d);
-        if (pos >= c.end(d)) {
+        const uint32_t pos = c.pos(d), end = c.end(d);
+        if (pos >= end) {
             if (adaptive) vis &= ~(1u << meta_vertex(c.meta(d)));
             --d;
             return d < floor ? kLaneExhausted : kLaneRunning;
         }
         c.pos(d) = pos + 1;
-        const uint32_t cand = a.pool(pos);
         const uint32_t m = c.meta(d);
         const uint32_t prim = meta_prim(m);
-        uint32_t w, nb, g0;
+        uint32_t w, nb, g0, sl;
         if (adaptive) {
             w = meta_vertex(m);
             nb = meta_secn(m) + 1;
             g0 = meta_secb(m);
+            sl = a.same_lab(w);
         } else {
             w = a.order(u, d);
             nb = a.bn_n(u, d);
             g0 = a.sec_base(u, d);
+            sl = a.same_before(u, d) | a.same_after(u, d);
         }
+        const uint32_t cand = pos == pf_pos ? pf_val : a.pool(pos);
+        // in flight beside this step's loads: the next primary entry and the
+        // candidate's vertex id
+        if (pos + 1 < end) {
+            pf_pos = pos + 1;
+            pf_val = a.pool(pos + 1);
+        }
+        const uint32_t v = sl ? a.vid(w, cand) : 0u;
         for (uint32_t k = 0, j = 0; k < nb; ++k) {
             if (k == prim) continue;
             const uint32_t g = g0 + j++;
-            uint32_t at, hi;
+            bool member;
             if (g < MV) {
-                hi = c.cend(g);
-                at = lane_gallop(a, c.cur(g), hi, cand);
+                const uint32_t cur = c.cur(g), hi = c.cend(g);
+                if (cur >= hi) return kLaneRunning;
+                uint32_t cv = c.curv(g), at = cur;
+                if (cv == kUnknown) at = lane_gallop(a, cur, hi, cand, cv);
+                else if (cv < cand) at = lane_gallop(a, cur + 1, hi, cand, cv);
                 c.cur(g) = at;
+                c.curv(g) = cv;
+                member = at < hi && cv == cand;
             } else {
                 // reference order only: more non-primary rows than cursor slots
-                uint32_t lo;
+                uint32_t lo, hi, cv;
                 const uint32_t s = a.bn_slot(u, d, k);
                 a.row(s, c.idx(a.slot_src(s)), lo, hi);
-                at = lane_gallop(a, lo, hi, cand);
+                const uint32_t at = lane_gallop(a, lo, hi, cand, cv);
+                member = at < hi && cv == cand;
             }
-            if (!(at < hi && a.pool(at) == cand)) return kLaneRunning;
+            if (!member) return kLaneRunning;
         }
-        if (adaptive) {
-            const uint32_t sl = a.same_lab(w);
-            if (sl) {
-                const uint32_t v = a.vid(w, cand);
+        if (sl) {
+            if (adaptive) {
                 for (uint32_t q = sl & vis & ~(1u << w); q; q &= q - 1)
                     if (c.vid(lane_ctz(q)) == v) return kLaneRunning;
                 c.vid(w) = v;
-            }
-        } else {
-            const uint32_t before = a.same_before(u, d);
-            if (before | a.same_after(u, d)) {
-                const uint32_t v = a.vid(w, cand);
-                for (uint32_t q = before; q; q &= q - 1)
+            } else {
+                for (uint32_t q = a.same_before(u, d); q; q &= q - 1)
                     if (c.vid(lane_ctz(q)) == v) return kLaneRunning;
                 c.vid(d) = v;
             }

@@ -434,6 +434,10 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
             const uint32_t slots = P.nv <= 8 ? 8u : 15u;
             P.adaptive = opt.match_order == TFSM_MATCH_ADAPTIVE && tp.num_edges <= tfsm_lane::kCutMax &&
                          tp.num_edges - (P.nv - 1) <= slots;
+            for (uint32_t e = 0; e < (uint32_t)kCutEdges; ++e) {
+                const uint32_t b = e < tp.num_edges ? (tp.edge_a[e] | tp.edge_b[e] << 4) : 0xffu;
+                P.edge_pack[e / 4] |= b << (8 * (e % 4));
+            }
         }
         distinct[p] = dl;
         acyclic[p] = tp.num_edges == tp.num_vertices - 1;
@@ -523,6 +527,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
             unsigned long long ntask = 0;
             if (decided[p] == TFSM_DECIDED_SEARCH) {
                 for (uint32_t sl = 0; sl < P.nslots; ++sl) {
+                    if (sl < 2u * kCutEdges) P.cut_ao[sl] = (uint32_t)ao;
                     P.aux_off[sl] = aux_region + ao;
                     ao += P.dom_size[P.slot_a[sl]] + 1;
                     aux_work.add(p, sl, P.dom_size[P.slot_a[sl]]);
@@ -576,6 +581,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
 
     // ---------------- existence search ----------------
     const unsigned long long total_tasks = task_base.back();
+    uint32_t verify_launches = 0;
     c->trace_len = 0;
     if (opt.trace) {
         if (total_tasks > c->trace_cap) {
@@ -596,6 +602,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         va.total = total_tasks;
         va.ctr = c->d_ctr;
         va.pool = c->d_pool;
+        va.aux_region = aux_region;
         va.trace = opt.trace ? c->d_trace : nullptr;
         va.steps_per_round = 4;
         va.split_steps = opt.split_steps == 0 ? kDefaultSplitSteps : opt.split_steps;
@@ -606,6 +613,9 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         va.piece_cnt = c->d_piece_cnt;
         va.piece_incomplete = c->d_piece_inc;
         va.order = nullptr;
+        va.order_pu = nullptr;
+        va.split = nullptr;
+        va.part = 0;
         va.abort_stopped = 0;
         if (total_tasks >= 0xffffffffull) throw ArgError{TFSM_ERR_LIMIT, "more than 2^32 tasks in one call"};
         // features (traced calls and the learned order), scores, priority sort
@@ -634,24 +644,37 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
             const uint32_t chunk = (uint32_t)((total_tasks + pg - 1) / pg);
             uint32_t *hist = c->work.alloc<uint32_t>((size_t)kPrioBins * pg);
             uint32_t *order = c->work.alloc<uint32_t>(total_tasks);
+            uint32_t *pu = c->work.alloc<uint32_t>(total_tasks);          // (pattern << 4) | vertex per task
+            uint32_t *order_pu = c->work.alloc<uint32_t>(total_tasks);
+            if (n >= (1u << 28)) throw ArgError{TFSM_ERR_LIMIT, "more than 2^28 patterns in one call"};
+            float *qsums = c->work.alloc<float>((size_t)n * kMaxV * kPrioQ);
+            uint8_t *cutoff = c->work.alloc<uint8_t>((size_t)n * kMaxV);
             HIPX(hipMemsetAsync(sums, 0, (size_t)n * kMaxV * sizeof(float), s));
+            HIPX(hipMemsetAsync(qsums, 0, (size_t)n * kMaxV * kPrioQ * sizeof(float), s));
             hipLaunchKernelGGL(k_mlp_forward, dim3(gt), dim3(kBlock), 0, s, feat, total_tasks, c->d_rank_w, logits);
             hipLaunchKernelGGL(k_group_sums, dim3(gt), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, logits,
-                               sums);
+                               sums, qsums);
+            hipLaunchKernelGGL(k_defer_cutoff, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d_ps, n, sums, qsums,
+                               cutoff);
             hipLaunchKernelGGL(k_priority, dim3(pg), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, chunk,
-                               logits, sums, keys, hist);
+                               logits, sums, cutoff, keys, pu, hist);
             const uint32_t nhist = kPrioBins * pg, ntiles = (nhist + kScanTile - 1) / kScanTile;
             static_assert((size_t)kPrioBins * 128 <= (size_t)kScanTile * kBlock, "scan of the sums is one block");
             uint32_t *tiles = c->work.alloc<uint32_t>(ntiles);
             hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, hist, nhist, tiles);
             hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, tiles, ntiles);
             hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(kBlock), 0, s, hist, nhist, tiles);
-            hipLaunchKernelGGL(k_prio_scatter, dim3(pg), dim3(kBlock), 0, s, keys, total_tasks, chunk, hist, order);
+            hipLaunchKernelGGL(k_prio_scatter, dim3(pg), dim3(kBlock), 0, s, keys, pu, total_tasks, chunk, hist, order,
+                               order_pu);
             HIPX(hipGetLastError());
             va.order = order;
+            va.order_pu = order_pu;
             va.abort_stopped = 1;
+            // after the scan, the first deferred bin of block 0 holds the
+            // number of first-part entries
+            va.split = hist + (size_t)kDeferredKey * pg;
         }
-        const uint32_t epoch = ++c->epoch;
+        uint32_t epoch = ++c->epoch;
         HIPX(hipMemsetAsync(c->d_ctr, 0, sizeof(SearchCounters), s));
         HIPX(hipEventRecord(c->ev[4], s));
         uint32_t max_nv = 0;
@@ -678,9 +701,25 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         // the second phase's blocks (4 waves each) all fit at once (20 waves per CU)
         const uint32_t long_blocks_per_cu =
             std::min(5u, std::max(1u, (opt.long_waves_per_cu ? opt.long_waves_per_cu : kDefaultLongWavesPerCu) / 4));
-        if (max_nv <= 8) hipLaunchKernelGGL(k_verify<8>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
-        else hipLaunchKernelGGL(k_verify<16>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
-        HIPX(hipGetLastError());
+        // the learned order runs in two launches: the tasks the ranker expects
+        // to decide the patterns, then the deferred rest (mostly skipped)
+        const bool two_parts = va.order && va.split && !va.long_cap;
+        auto launch_verify = [&]() {
+            if (max_nv <= 8) hipLaunchKernelGGL(k_verify<8>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
+            else hipLaunchKernelGGL(k_verify<16>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
+            HIPX(hipGetLastError());
+        };
+        va.part = two_parts ? 1u : 0u;
+        launch_verify();
+        if (two_parts) {
+            // the first launch's counters (splits, watchdog) before they are cleared
+            HIPX(hipMemcpyAsync(c->h_ctr.ensure(2) + 1, c->d_ctr, sizeof(SearchCounters), hipMemcpyDeviceToHost, s));
+            epoch = ++c->epoch;
+            HIPX(hipMemsetAsync(c->d_ctr, 0, sizeof(SearchCounters), s));
+            va.part = 2;
+            launch_verify();
+        }
+        verify_launches = two_parts ? 2u : 1u;
         if (va.long_cap) {
             hipLaunchKernelGGL(k_verify_long<8>, dim3((uint32_t)c->num_cus * long_blocks_per_cu), dim3(kBlock), 0,
                                s, va, epoch);
@@ -693,7 +732,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
     }
     HIPX(hipEventRecord(c->ev[3], s));
     HIPX(hipMemcpyAsync(hst, d_st, n * sizeof(PatState), hipMemcpyDeviceToHost, s));
-    SearchCounters *hctr = c->h_ctr.ensure(1);
+    SearchCounters *hctr = c->h_ctr.ensure(2);
     if (total_tasks) HIPX(hipMemcpyAsync(hctr, c->d_ctr, sizeof(SearchCounters), hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
     std::memset(&c->stats, 0, sizeof(c->stats));
@@ -701,10 +740,14 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
     HIPX(hipEventElapsedTime(&c->stats.table_ms, c->ev[1], c->ev[2]));
     if (total_tasks) {
         HIPX(hipEventElapsedTime(&c->stats.search_ms, c->ev[4], c->ev[5]));
-        c->stats.search_launches = 1;
+        c->stats.search_launches = verify_launches;
         HIPX(hipEventElapsedTime(&c->stats.rank_ms, c->ev[2], c->ev[4]));
         c->stats.search_splits = std::min(hctr->split_tail, kSplitCap);
         c->stats.search_watchdog = hctr->watchdog ? 1u : 0u;
+        if (verify_launches == 2) {
+            c->stats.search_splits += std::min(hctr[1].split_tail, kSplitCap);
+            if (hctr[1].watchdog) c->stats.search_watchdog = 1;
+        }
     }
     {
         // algorithmic bytes (definitions in tfsm_gpu.h)

@@ -45,11 +45,11 @@ namespace {
 constexpr int kMV = 8;
 
 struct HostCols {
-    uint32_t idx_[kMV], vid_[kMV], pos_[kMV], end_[kMV], cur_[kMV], cend_[kMV], meta_[kMV];
+    uint32_t idx_[kMV], vid_[kMV], pos_[kMV], end_[kMV], cur_[kMV], cend_[kMV], curv_[kMV], meta_[kMV];
     uint32_t n = 0;    // pattern vertices: idx and vid by vertex, the rest by depth < n
     void poison() {
         for (int i = 0; i < kMV; ++i)
-            idx_[i] = vid_[i] = pos_[i] = end_[i] = cur_[i] = cend_[i] = meta_[i] = 0xdeadbeefu;
+            idx_[i] = vid_[i] = pos_[i] = end_[i] = cur_[i] = cend_[i] = curv_[i] = meta_[i] = 0xdeadbeefu;
     }
     uint32_t &at(uint32_t *a, uint32_t i, uint32_t lim, const char *what) {
         if (i >= lim) violation(what, i, lim);
@@ -61,6 +61,7 @@ struct HostCols {
     uint32_t &end(uint32_t d) { return at(end_, d, n, "end"); }
     uint32_t &cur(uint32_t g) { return at(cur_, g, kMV, "cur"); }
     uint32_t &cend(uint32_t g) { return at(cend_, g, kMV, "cend"); }
+    uint32_t &curv(uint32_t g) { return at(curv_, g, kMV, "curv"); }
     uint32_t &meta(uint32_t d) { return at(meta_, d, n, "meta"); }
 };
 
@@ -133,7 +134,6 @@ struct HostAcc {
             if (lab[ord[u][j]] == lab[ord[u][d]]) return 1;
         return 0;
     }
-    uint32_t ne() const { return (uint32_t)edges.size(); }
     uint32_t edge_x(uint32_t e) const {
         if (e >= edges.size()) violation("edge_x", e, edges.size());
         return edges[e].first;
@@ -142,8 +142,19 @@ struct HostAcc {
         if (e >= edges.size()) violation("edge_y", e, edges.size());
         return edges[e].second;
     }
-    uint32_t slot_xy(uint32_t e) const { return edge_x(e) * n + edge_y(e); }
-    uint32_t slot_yx(uint32_t e) const { return edge_y(e) * n + edge_x(e); }
+    uint32_t edge_pack(uint32_t i) const {
+        if (i >= tfsm_lane::kCutMax / 4) violation("edge_pack", i, tfsm_lane::kCutMax / 4);
+        uint32_t w = 0;
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t e = 4 * i + k;
+            const uint32_t b = e < edges.size() ? (edges[e].first | edges[e].second << 4) : 0xffu;
+            w |= b << (8 * k);
+        }
+        return w;
+    }
+    void edge_row(uint32_t e, bool from_x, uint32_t r, uint32_t &lo, uint32_t &hi) const {
+        row(from_x ? edge_x(e) * n + edge_y(e) : edge_y(e) * n + edge_x(e), r, lo, hi);
+    }
     uint32_t same_lab(uint32_t x) const {
         if (x >= n) violation("same_lab", x, n);
         uint32_t m = 0;
