@@ -630,11 +630,10 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
             }
             c->feat_len = total_tasks;
             feat = c->d_feat;
-        } else if (learned) {
-            feat = c->work.alloc<float>(total_tasks * kFeat);
         }
-        if (feat) hipLaunchKernelGGL(k_features, dim3(gt), dim3(kBlock), 0, s, c->g, d_ps, d_st, va.task_base, n,
-                                     total_tasks, feat);
+        if (opt.trace)
+            hipLaunchKernelGGL(k_features, dim3(gt), dim3(kBlock), 0, s, c->g, d_ps, d_st, va.task_base, n,
+                               total_tasks, feat);
         if (learned) {
             float *logits = c->work.alloc<float>(total_tasks);
             float *sums = c->work.alloc<float>((size_t)n * kMaxV);
@@ -651,9 +650,16 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
             uint8_t *cutoff = c->work.alloc<uint8_t>((size_t)n * kMaxV);
             HIPX(hipMemsetAsync(sums, 0, (size_t)n * kMaxV * sizeof(float), s));
             HIPX(hipMemsetAsync(qsums, 0, (size_t)n * kMaxV * kPrioQ * sizeof(float), s));
-            hipLaunchKernelGGL(k_mlp_forward, dim3(gt), dim3(kBlock), 0, s, feat, total_tasks, c->d_rank_w, logits);
-            hipLaunchKernelGGL(k_group_sums, dim3(gt), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, logits,
-                               sums, qsums);
+            if (feat) {
+                // traced: the feature rows are kept for training
+                hipLaunchKernelGGL(k_mlp_forward, dim3(gt), dim3(kBlock), 0, s, feat, total_tasks, c->d_rank_w,
+                                   logits);
+                hipLaunchKernelGGL(k_group_sums, dim3(gt), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks,
+                                   logits, sums, qsums);
+            } else {
+                hipLaunchKernelGGL(k_rank_scores, dim3(gt), dim3(kBlock), 0, s, c->g, d_ps, va.task_base, n,
+                                   total_tasks, c->d_rank_w, logits, sums, qsums);
+            }
             hipLaunchKernelGGL(k_defer_cutoff, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d_ps, n, sums, qsums,
                                cutoff);
             hipLaunchKernelGGL(k_priority, dim3(pg), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, chunk,
