@@ -142,6 +142,39 @@ struct Pinned {
     }
 };
 
+// Pinned host staging for one call's uploads: each upload is copied here and
+// sent with an asynchronous copy (from pageable memory the runtime would
+// stage it synchronously).  Chunks are only added during a call and reused
+// by the next one, which starts after the previous call's final sync.
+struct PinnedStage {
+    std::vector<std::pair<char *, size_t>> chunks;
+    size_t cur = 0, off = 0;
+    ~PinnedStage() {
+        for (auto &ch : chunks) (void)hipHostFree(ch.first);
+    }
+    void reset() { cur = off = 0; }
+    void *alloc(size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        while (cur < chunks.size() && off + bytes > chunks[cur].second) {
+            ++cur;
+            off = 0;
+        }
+        if (cur == chunks.size()) {
+            const size_t sz = std::max<size_t>(bytes, 4u << 20);
+            char *p = nullptr;
+            HIPX(hipHostMalloc(reinterpret_cast<void **>(&p), sz, hipHostMallocDefault));
+            chunks.emplace_back(p, sz);
+            off = 0;
+        }
+        void *r = chunks[cur].first + off;
+        off += bytes;
+        return r;
+    }
+};
+
+constexpr uint32_t kMaxPrunePasses = 4096;   // filter passes of one call
+constexpr uint32_t kPruneBatch = 6;          // passes launched between convergence checks
+
 }  // namespace
 
 struct tfsm_ctx {
@@ -159,7 +192,9 @@ struct tfsm_ctx {
     Arena work;
     uint32_t *d_pool = nullptr;
     uint32_t pool_cap = 0;
-    uint32_t *d_flags = nullptr;             // [0] pool_used, [1] overflow, [2..] prune changed flags
+    uint32_t *d_flags = nullptr;             // [0] pool_used, [1] overflow
+    uint32_t *d_pflags = nullptr;            // per filter pass: 1 when it removed a candidate
+    PinnedStage stage;                       // upload staging of the current call
     SearchCounters *d_ctr = nullptr;
     SplitRecord *d_records = nullptr;        // kSplitCap records, reused across launches by epoch
     uint32_t *d_piece_cnt = nullptr, *d_piece_inc = nullptr;
@@ -214,10 +249,17 @@ uint32_t grid_for(unsigned long long items, int num_cus) {
     return (uint32_t)std::max<unsigned long long>(1, std::min(blocks, cap));
 }
 
+// Host data to device memory `d` through the pinned staging of the call.
+void stage_copy(tfsm_ctx *c, void *d, const void *h, size_t bytes) {
+    void *st = c->stage.alloc(bytes);
+    std::memcpy(st, h, bytes);
+    HIPX(hipMemcpyAsync(d, st, bytes, hipMemcpyHostToDevice, c->stream));
+}
+
 template <class T>
 T *upload(tfsm_ctx *c, const std::vector<T> &v) {
     T *d = c->work.alloc<T>(std::max<size_t>(v.size(), 1));
-    if (!v.empty()) HIPX(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    if (!v.empty()) stage_copy(c, d, v.data(), v.size() * sizeof(T));
     return d;
 }
 
@@ -360,6 +402,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
     }
     c->work.reset();
     hipStream_t s = c->stream;
+    HIPX(hipStreamSynchronize(s));      // nothing of an earlier call still reads the staging
+    c->stage.reset();
     HIPX(hipEventRecord(c->ev[0], s));
 
     // ---------------- host description of every pattern ----------------
@@ -457,7 +501,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
     }
     PatStatic *d_ps = c->work.alloc<PatStatic>(n);
     PatState *d_st = c->work.alloc<PatState>(n);
-    HIPX(hipMemcpyAsync(d_ps, hs.data(), n * sizeof(PatStatic), hipMemcpyHostToDevice, s));
+    stage_copy(c, d_ps, hs.data(), n * sizeof(PatStatic));
     HIPX(hipMemsetAsync(d_st, 0, n * sizeof(PatState), s));
 
     // ---------------- filter: LDF, DP-iso to the fixpoint, compaction ----------------
@@ -475,21 +519,28 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
     if (wsrc.total) hipLaunchKernelGGL(k_ldf, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc);
     HIPX(hipGetLastError());
     uint32_t *hf = c->h_flags.ensure(8);
-    for (int round = 0; wsrc.total; ++round) {
-        HIPX(hipMemsetAsync(c->d_flags + 2, 0, 2 * sizeof(uint32_t), s));
-        hipLaunchKernelGGL(k_prune, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc, c->d_flags + 2);
-        hipLaunchKernelGGL(k_prune, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc, c->d_flags + 3);
+    // DP-iso passes in batches without host round trips: a pass returns at
+    // once when the one before it removed nothing (the fixpoint is reached);
+    // the last flag of a batch, read with the compacted domains, says whether
+    // another batch is needed (compaction is then simply redone)
+    PatState *hst = c->h_state.ensure(n);
+    uint32_t passes = 0;
+    if (wsrc.total) HIPX(hipMemsetAsync(c->d_pflags, 0, kMaxPrunePasses * sizeof(uint32_t), s));
+    for (;;) {
+        for (uint32_t i = 0; wsrc.total && i < kPruneBatch; ++i, ++passes) {
+            if (passes >= kMaxPrunePasses) throw ArgError{TFSM_ERR_LIMIT, "filter did not converge"};
+            hipLaunchKernelGGL(k_prune, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc, c->d_pflags + passes,
+                               passes ? c->d_pflags + passes - 1 : nullptr);
+        }
+        hipLaunchKernelGGL(k_compact, dim3((uint32_t)pair_p.size()), dim3(kBlock), 0, s, c->g, d_ps, d_st, d_pair_p,
+                           d_pair_u);
         HIPX(hipGetLastError());
-        HIPX(hipMemcpyAsync(hf, c->d_flags + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPX(hipMemcpyAsync(hst, d_st, n * sizeof(PatState), hipMemcpyDeviceToHost, s));
+        hf[0] = 0;
+        if (passes) HIPX(hipMemcpyAsync(hf, c->d_pflags + passes - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIPX(hipStreamSynchronize(s));
         if (hf[0] == 0) break;
-        if (round > 100000) throw ArgError{TFSM_ERR_LIMIT, "filter did not converge"};
     }
-    hipLaunchKernelGGL(k_compact, dim3((uint32_t)pair_p.size()), dim3(kBlock), 0, s, c->g, d_ps, d_st, d_pair_p, d_pair_u);
-    HIPX(hipGetLastError());
-    PatState *hst = c->h_state.ensure(n);
-    HIPX(hipMemcpyAsync(hst, d_st, n * sizeof(PatState), hipMemcpyDeviceToHost, s));
-    HIPX(hipStreamSynchronize(s));
     HIPX(hipEventRecord(c->ev[1], s));
 
     // ---------------- decisions and plans ----------------
@@ -544,7 +595,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
             task_base.push_back(task_base.back() + ntask);
         }
     }
-    HIPX(hipMemcpyAsync(d_ps, hs.data(), n * sizeof(PatStatic), hipMemcpyHostToDevice, s));
+    stage_copy(c, d_ps, hs.data(), n * sizeof(PatStatic));
 
     // ---------------- auxiliary tables ----------------
     if (!slot_p.empty()) {
@@ -575,8 +626,11 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
             HIPX(hipMalloc(&c->d_pool, (size_t)c->pool_cap * sizeof(uint32_t)));
         }
     }
-    unsigned long long scanned = 0;
-    if (!slot_p.empty()) HIPX(hipMemcpy(&scanned, c->d_scanned, sizeof(scanned), hipMemcpyDeviceToHost));
+    // neighbour ids the table build scanned, read with the final results
+    unsigned long long *h_scanned = reinterpret_cast<unsigned long long *>(c->stage.alloc(sizeof(unsigned long long)));
+    *h_scanned = 0;
+    if (!slot_p.empty())
+        HIPX(hipMemcpyAsync(h_scanned, c->d_scanned, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPX(hipEventRecord(c->ev[2], s));
 
     // ---------------- existence search ----------------
@@ -757,7 +811,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
     }
     {
         // algorithmic bytes (definitions in tfsm_gpu.h)
-        unsigned long long tb = 4ull * scanned, sb = 0;
+        unsigned long long tb = 4ull * *h_scanned, sb = 0;
         for (uint32_t p = 0; p < n; ++p) {
             if (decided[p] != TFSM_DECIDED_SEARCH) continue;
             const PatStatic &P = hs[p];
@@ -942,6 +996,7 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         c->pool_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(4 * g->num_adj, 1u << 22), 0xfffffff0ull);
         HIPX(hipMalloc(&c->d_pool, (size_t)c->pool_cap * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_flags, 16 * sizeof(uint32_t)));
+        HIPX(hipMalloc(&c->d_pflags, kMaxPrunePasses * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_ctr, sizeof(SearchCounters)));
         HIPX(hipMalloc(&c->d_records, (size_t)kSplitCap * sizeof(SplitRecord)));
         HIPX(hipMalloc(&c->d_piece_cnt, (size_t)kSplitCap * sizeof(uint32_t)));
@@ -975,6 +1030,7 @@ int tfsm_ctx_destroy(tfsm_ctx *c) {
     (void)hipFree(c->d_piece_cnt);
     (void)hipFree(c->d_piece_inc);
     (void)hipFree(c->d_scanned);
+    (void)hipFree(c->d_pflags);
     (void)hipFree(c->d_rank_w);
     (void)hipFree(c->d_feat);
     (void)hipFree(c->d_trace);
