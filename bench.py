@@ -79,6 +79,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--paths", default="plain,batched,learned",
+                    help="paths to time (comma list of plain, batched, learned; the value needs learned)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first copy this many MiB device to device (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards across patterns "
@@ -121,15 +125,30 @@ def main():
         parity = gold is None or keys == gold
         return elapsed, agg, parity, n
 
-    plain_s, plain_agg, plain_ok, _ = run(False, False, args.steps, args.warmup, "plain")
-    batch_s, batch_agg, batch_ok, _ = run(True, False, args.steps, args.warmup, "batched")
+    if args.copy_mib:
+        src = torch.empty(args.copy_mib << 18, dtype=torch.int32, device="cuda")
+        src.fill_(1)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+    paths = set(args.paths.split(","))
+    none = (float("nan"), None, True, 0)
+    plain_s, plain_agg, plain_ok, _ = run(False, False, args.steps, args.warmup, "plain") if "plain" in paths else none
+    batch_s, batch_agg, batch_ok, _ = run(True, False, args.steps, args.warmup, "batched") if "batched" in paths else none
     learn_s, agg, learn_ok, nfreq = run(True, True, args.steps, args.warmup, "learned")
 
     K = args.steps
     evaluated = agg["evaluated"] / K
     value = agg["evaluated"] / learn_s
-    plain_value = plain_agg["evaluated"] / plain_s
-    batch_value = batch_agg["evaluated"] / batch_s
+    plain_value = plain_agg["evaluated"] / plain_s if plain_agg else None
+    batch_value = batch_agg["evaluated"] / batch_s if batch_agg else None
+
+    def side(a, sec, v):
+        if not a:
+            return None
+        return {"value": round(v, 3), "ms_per_step": round(1e3 * sec / K, 3),
+                "verifications_per_s": round(a["verdicts"] / sec, 1), "searches_per_step": a["searches"] / K,
+                "search_ms_per_step": round(a["search_ms"] / K, 3)}
     ms_search = agg["search_ms"] / max(1, agg["search_launches"])
     bytes_per_launch = agg["bytes_search"] / max(1, agg["search_launches"])
     achieved = bytes_per_launch / (ms_search * 1e-3) / 1e9 if ms_search > 0 else 0.0
@@ -170,17 +189,11 @@ def main():
         "frequent_patterns": nfreq,
         "searches_per_step": agg["searches"] / K,
         "pattern_set_matches_reference": bool(learn_ok and batch_ok and plain_ok),
-        "plain": {"value": round(plain_value, 3), "ms_per_step": round(1e3 * plain_s / K, 3),
-                  "verifications_per_s": round(plain_agg["verdicts"] / plain_s, 1),
-                  "searches_per_step": plain_agg["searches"] / K,
-                  "search_ms_per_step": round(plain_agg["search_ms"] / K, 3)},
-        "batched": {"value": round(batch_value, 3), "ms_per_step": round(1e3 * batch_s / K, 3),
-                    "verifications_per_s": round(batch_agg["verdicts"] / batch_s, 1),
-                    "searches_per_step": batch_agg["searches"] / K,
-                    "search_ms_per_step": round(batch_agg["search_ms"] / K, 3)},
+        "plain": side(plain_agg, plain_s, plain_value),
+        "batched": side(batch_agg, batch_s, batch_value),
         "search_ms_per_step": round(agg["search_ms"] / K, 3),
         "speedup_vs_plain": round(value / plain_value, 3) if plain_value else None,
-        "speedup_batched_vs_plain": round(batch_value / plain_value, 3) if plain_value else None,
+        "speedup_batched_vs_plain": round(batch_value / plain_value, 3) if plain_value and batch_value else None,
         "roofline": {"bound": "hbm", "kernel": "k_verify (existence search)", "achieved": round(achieved, 3),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 6),
                      "traffic": None, "launches": agg["search_launches"] // K,
