@@ -4,9 +4,13 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <thread>
 
 namespace tfsm {
@@ -52,20 +56,93 @@ bool to_device_pattern(const DataGraph &g, const Pattern &p, tfsm_pattern &out) 
     return true;
 }
 
+// Worker threads kept for the life of the process: a mining run grows
+// several levels, and starting threads for each of them cost more than the
+// growth of a small level.  Each job has its own state; the caller takes
+// items too and waits only for the items, so a worker that wakes late finds
+// its job drained and does nothing.
+class WorkerPool {
+public:
+    static WorkerPool &get() {
+        static WorkerPool pool;
+        return pool;
+    }
+    ~WorkerPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : workers_) t.join();
+    }
+    void run(size_t n, int threads, const std::function<void(size_t)> &fn) {
+        std::lock_guard<std::mutex> serial(run_mu_);
+        auto job = std::make_shared<Job>();
+        job->fn = &fn;
+        job->n = n;
+        job->left.store(n);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            while ((int)workers_.size() < threads - 1) workers_.emplace_back([this]() { loop(); });
+            cur_ = job;
+            ++gen_;
+        }
+        cv_.notify_all();
+        drain(*job);
+        std::unique_lock<std::mutex> lk(job->m);
+        job->cv.wait(lk, [&]() { return job->left.load() == 0; });
+        std::lock_guard<std::mutex> lk2(mu_);
+        cur_.reset();
+    }
+
+private:
+    struct Job {
+        const std::function<void(size_t)> *fn = nullptr;
+        size_t n = 0;
+        std::atomic<size_t> next{0}, left{0};
+        std::mutex m;
+        std::condition_variable cv;
+    };
+    static void drain(Job &job) {
+        size_t i;
+        while ((i = job.next.fetch_add(1)) < job.n) {
+            (*job.fn)(i);
+            if (job.left.fetch_sub(1) == 1) {
+                std::lock_guard<std::mutex> lk(job.m);
+                job.cv.notify_all();
+            }
+        }
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            std::shared_ptr<Job> job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&]() { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                job = cur_;
+            }
+            if (job) drain(*job);
+        }
+    }
+    std::mutex run_mu_, mu_;
+    std::condition_variable cv_;
+    std::vector<std::thread> workers_;
+    std::shared_ptr<Job> cur_;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
 template <class F>
 void parallel_for(size_t n, int threads, F &&f) {
     if (threads <= 1 || n < 2) {
         for (size_t i = 0; i < n; ++i) f(i);
         return;
     }
-    std::atomic<size_t> next{0};
-    std::vector<std::thread> pool;
-    for (int t = 0; t < threads; ++t)
-        pool.emplace_back([&]() {
-            size_t i;
-            while ((i = next.fetch_add(1)) < n) f(i);
-        });
-    for (auto &t : pool) t.join();
+    const std::function<void(size_t)> fn = [&](size_t i) { f(i); };
+    WorkerPool::get().run(n, std::min<int>(threads, (int)n), fn);
 }
 
 }  // namespace
