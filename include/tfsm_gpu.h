@@ -135,6 +135,12 @@ typedef struct tfsm_ctx tfsm_ctx;
  * Replaces Worker::load_data's hand-off of grami.pruned_graph (worker.h:47-55). */
 int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out);
 int tfsm_ctx_destroy(tfsm_ctx *ctx);
+/* Capacity of the device pool that holds the auxiliary tables (entries of 4
+ * bytes; the default is max(4 x adjacency entries, 2^22)).  It still grows on
+ * demand: a call whose tables do not fit is redone with the pool enlarged.
+ * No counterpart in the reference, which allocates its tables per pattern
+ * (gmatch.h:995-1130). */
+int tfsm_ctx_set_pool_capacity(tfsm_ctx *ctx, uint64_t entries);
 
 /* One pattern, one launch sequence (comper.h:1655-1700 + 431-533 + 1181-1300).
  * parent_handle: retained domains of the DFS-code parent (0: label buckets, as

@@ -152,6 +152,7 @@ struct VerifyArgs {
     SearchCounters *ctr;
     const uint32_t *pool;
     const uint32_t *aux_region;            // every slot's row offsets of the call
+    const uint32_t *overflow;              // nonzero: the tables did not fit the pool (do nothing)
     uint32_t *trace;                       // nullable, indexed by reference task order
     const uint32_t *order;                 // nullable: search order -> reference task index
     const uint32_t *order_pu;              // with order: (pattern << 4) | pattern vertex of each entry

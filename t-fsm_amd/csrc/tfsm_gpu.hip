@@ -376,7 +376,7 @@ struct EvalCall {
     tfsm_pattern_result *out;
 };
 
-int run_eval(tfsm_ctx *c, const EvalCall &call) {
+int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     const uint32_t n = call.n;
     const tfsm_eval_options &opt = call.opt;
     if (opt.mode != TFSM_MODE_CAPPED && opt.mode != TFSM_MODE_EXACT)
@@ -526,6 +526,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
     PatState *hst = c->h_state.ensure(n);
     uint32_t passes = 0;
     if (wsrc.total) HIPX(hipMemsetAsync(c->d_pflags, 0, kMaxPrunePasses * sizeof(uint32_t), s));
+    HIPX(hipMemsetAsync(c->d_flags, 0, 2 * sizeof(uint32_t), s));     // pool used, pool overflow
     for (;;) {
         for (uint32_t i = 0; wsrc.total && i < kPruneBatch; ++i, ++passes) {
             if (passes >= kMaxPrunePasses) throw ArgError{TFSM_ERR_LIMIT, "filter did not converge"};
@@ -602,29 +603,18 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         WorkTable waux = aux_work.to_device(c);
         const uint32_t *d_slot_p = upload(c, slot_p), *d_slot_s = upload(c, slot_s);
         const uint32_t gaux = grid_for(waux.total, c->num_cus);
-        for (int attempt = 0;; ++attempt) {
-            HIPX(hipMemsetAsync(c->d_flags, 0, 2 * sizeof(uint32_t), s));
-            HIPX(hipMemsetAsync(c->d_scanned, 0, sizeof(unsigned long long), s));
-            if (waux.total)
-                hipLaunchKernelGGL(k_aux_count, dim3(gaux), dim3(kBlock), 0, s, c->g, d_ps, waux, c->d_scanned);
-            hipLaunchKernelGGL(k_aux_scan, dim3((uint32_t)slot_p.size()), dim3(kBlock), 0, s, d_ps, d_st, d_slot_p,
-                               d_slot_s, c->d_flags, c->pool_cap, c->d_flags + 1);
-            if (waux.total)
-                hipLaunchKernelGGL(k_aux_fill, dim3(gaux), dim3(kBlock), 0, s, c->g, d_ps, d_st, waux, c->d_pool,
-                                   c->d_flags + 1);
-            HIPX(hipGetLastError());
-            HIPX(hipMemcpyAsync(hf, c->d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-            HIPX(hipStreamSynchronize(s));
-            if (!hf[1]) break;
-            if (attempt > 4) throw ArgError{TFSM_ERR_LIMIT, "table pool overflow"};
-            // grow the pool to the measured need and rebuild
-            unsigned long long need = (unsigned long long)hf[0] * 2;
-            if (need > 0xfffffff0ull) throw ArgError{TFSM_ERR_LIMIT, "tables exceed 2^32 entries"};
-            HIPX(hipFree(c->d_pool));
-            c->d_pool = nullptr;
-            c->pool_cap = (uint32_t)need;
-            HIPX(hipMalloc(&c->d_pool, (size_t)c->pool_cap * sizeof(uint32_t)));
-        }
+        // no host round trip: an overflow of the pool is found with the
+        // final results (the search kernels do nothing on an overflowed
+        // pool), and the call is then redone with a larger pool
+        HIPX(hipMemsetAsync(c->d_scanned, 0, sizeof(unsigned long long), s));
+        if (waux.total)
+            hipLaunchKernelGGL(k_aux_count, dim3(gaux), dim3(kBlock), 0, s, c->g, d_ps, waux, c->d_scanned);
+        hipLaunchKernelGGL(k_aux_scan, dim3((uint32_t)slot_p.size()), dim3(kBlock), 0, s, d_ps, d_st, d_slot_p,
+                           d_slot_s, c->d_flags, c->pool_cap, c->d_flags + 1);
+        if (waux.total)
+            hipLaunchKernelGGL(k_aux_fill, dim3(gaux), dim3(kBlock), 0, s, c->g, d_ps, d_st, waux, c->d_pool,
+                               c->d_flags + 1);
+        HIPX(hipGetLastError());
     }
     // neighbour ids the table build scanned, read with the final results
     unsigned long long *h_scanned = reinterpret_cast<unsigned long long *>(c->stage.alloc(sizeof(unsigned long long)));
@@ -657,6 +647,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
         va.ctr = c->d_ctr;
         va.pool = c->d_pool;
         va.aux_region = aux_region;
+        va.overflow = c->d_flags + 1;
         va.trace = opt.trace ? c->d_trace : nullptr;
         va.steps_per_round = 4;
         va.split_steps = opt.split_steps == 0 ? kDefaultSplitSteps : opt.split_steps;
@@ -794,7 +785,20 @@ int run_eval(tfsm_ctx *c, const EvalCall &call) {
     HIPX(hipMemcpyAsync(hst, d_st, n * sizeof(PatState), hipMemcpyDeviceToHost, s));
     SearchCounters *hctr = c->h_ctr.ensure(2);
     if (total_tasks) HIPX(hipMemcpyAsync(hctr, c->d_ctr, sizeof(SearchCounters), hipMemcpyDeviceToHost, s));
+    uint32_t *hpool = hf + 4;                // pool used, pool overflow
+    HIPX(hipMemcpyAsync(hpool, c->d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
+    if (hpool[1]) {
+        // the tables did not fit: grow the pool to twice the measured need and redo the call
+        if (attempt >= 4) throw ArgError{TFSM_ERR_LIMIT, "table pool overflow"};
+        const unsigned long long need = (unsigned long long)hpool[0] * 2;
+        if (need > 0xfffffff0ull) throw ArgError{TFSM_ERR_LIMIT, "tables exceed 2^32 entries"};
+        HIPX(hipFree(c->d_pool));
+        c->d_pool = nullptr;
+        c->pool_cap = (uint32_t)std::max<unsigned long long>(need, c->pool_cap);
+        HIPX(hipMalloc(&c->d_pool, (size_t)c->pool_cap * sizeof(uint32_t)));
+        return run_eval(c, call, attempt + 1);
+    }
     std::memset(&c->stats, 0, sizeof(c->stats));
     HIPX(hipEventElapsedTime(&c->stats.filter_ms, c->ev[0], c->ev[1]));
     HIPX(hipEventElapsedTime(&c->stats.table_ms, c->ev[1], c->ev[2]));
@@ -1013,6 +1017,19 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
     }
     *out = c;
     return TFSM_OK;
+}
+
+int tfsm_ctx_set_pool_capacity(tfsm_ctx *c, uint64_t entries) {
+    if (!c || entries == 0) return TFSM_ERR_ARGUMENT;
+    if (entries > 0xfffffff0ull) return TFSM_ERR_LIMIT;
+    return guarded(c, [&]() -> int {
+        HIPX(hipStreamSynchronize(c->stream));
+        HIPX(hipFree(c->d_pool));
+        c->d_pool = nullptr;
+        c->pool_cap = (uint32_t)entries;
+        HIPX(hipMalloc(&c->d_pool, (size_t)c->pool_cap * sizeof(uint32_t)));
+        return TFSM_OK;
+    });
 }
 
 int tfsm_ctx_destroy(tfsm_ctx *c) {

@@ -92,7 +92,7 @@ class RankerWeights(ctypes.Structure):
 
 
 # every symbol include/tfsm_gpu.h declares
-GPU_SYMBOLS = ["tfsm_ctx_create", "tfsm_ctx_destroy", "tfsm_eval_pattern", "tfsm_eval_level",
+GPU_SYMBOLS = ["tfsm_ctx_create", "tfsm_ctx_destroy", "tfsm_ctx_set_pool_capacity", "tfsm_eval_pattern", "tfsm_eval_level",
                "tfsm_domain_read", "tfsm_domain_release", "tfsm_last_domain", "tfsm_last_states",
                "tfsm_last_aux", "tfsm_last_order", "tfsm_read_trace", "tfsm_last_stats",
                "tfsm_status_string", "tfsm_last_error", "tfsm_ranker_set", "tfsm_ranker_get",
@@ -118,6 +118,7 @@ def gpu_lib():
         vp = ctypes.c_void_p
         lib.tfsm_ctx_create.argtypes = [P(Csr), P(vp)]
         lib.tfsm_ctx_destroy.argtypes = [vp]
+        lib.tfsm_ctx_set_pool_capacity.argtypes = [vp, ctypes.c_uint64]
         lib.tfsm_eval_pattern.argtypes = [vp, P(Pattern), ctypes.c_uint64, ctypes.c_uint32, P(EvalOptions),
                                           P(PatternResult)]
         lib.tfsm_eval_level.argtypes = [vp, P(Pattern), P(ctypes.c_uint64), ctypes.c_uint32, ctypes.c_uint32,
@@ -285,6 +286,9 @@ class Context:
                           task_order=ORDER_LEARNED if learned else ORDER_REFERENCE, trace=int(trace), **tuning)
         _check(self.h, self.lib.tfsm_eval_level(self.h, arr, par, n, int(freq), ctypes.byref(opt), res))
         return list(res)
+
+    def set_pool_capacity(self, entries):
+        _check(self.h, self.lib.tfsm_ctx_set_pool_capacity(self.h, ctypes.c_uint64(int(entries))))
 
     def last_domain(self, u):
         import numpy as np

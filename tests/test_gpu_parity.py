@@ -284,6 +284,24 @@ def test_split_level_parity(workdir):
         assert list(r.valid_count) == list(s.valid_count) and list(r.invalid_count) == list(s.invalid_count)
 
 
+def test_tables_outgrow_the_pool(workdir):
+    """With a table pool far too small, a call's tables overflow it; the call is
+    redone with the pool enlarged, and the results equal the oracle's (a
+    single pattern and a whole level)."""
+    env = Env(workload_path("gnm2k", 1, workdir), 40)
+    env.ctx.set_pool_capacity(256)
+    cases = variants(golden_keys("gnm2k_f40_m4"), 12)
+    for vl, es in cases:
+        check_exact(env, vl, es)
+    env.ctx.set_pool_capacity(256)
+    pats = [p for p in (env.dense(vl, es) for vl, es in cases) if p is not None]
+    lev = env.ctx.eval_level(pats, env.freq, mode=tfsm.MODE_EXACT)
+    for (vl, es), r in zip([c for c in cases if env.dense(*c) is not None], lev):
+        ref = env.og.eval(vl, es, env.freq, mode=1)
+        assert (r.support_lower, r.support_upper) == (ref.lower, ref.upper), f"{vl} {es}"
+    env.ctx.close()
+
+
 def test_large_pattern_and_errors(workdir):
     """Nine-vertex paths use the 16-deep search kernel; malformed input is an error code."""
     env = env_for("gnm2k", 1, 40, workdir)
