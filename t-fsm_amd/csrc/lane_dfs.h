@@ -102,9 +102,11 @@ TFSM_LANE_FN uint32_t lane_gallop(const A &a, uint32_t lo, uint32_t hi, uint32_t
 // bounds of row r of a slot's table), reference plan order(u, d), bn_n(u, d),
 // bn_slot(u, d, k), slot_src(s), sec_base(u, d), same_before(u, d),
 // same_after(u, d); adaptive order: edge_pack(i) (pattern edges, one byte
-// each, x | y << 4, four to a word, 0xff past the last), edge_row(e, from_x,
-// r, lo, hi) (row r of edge e's table from x, over C(x), or from y),
-// same_lab(x) (other vertices with x's label).
+// each, x | y << 4, four to a word, 0xff past the last), cut_ao(s) (where
+// the row offsets of slot s start; slot 2e is edge e's table from x, over
+// C(x), and 2e + 1 the one from y) and row_at(ao, r, lo, hi) (row r of the
+// table whose offsets start at ao), same_lab(x) (other vertices with x's
+// label).
 // C: idx(x), vid(j), pos(d), end(d), cur(g), cend(g), curv(g), meta(d), all
 // lvalues.
 template <int MV>
@@ -167,46 +169,54 @@ struct Lane {
     // one with the shortest cut row) with primary row `prim` (the shortest
     // of w's rows when kNone).  w's backward rows are numbered in pattern
     // edge order, which a piece's receiver repeats.  The edges come from
-    // registers; every cut edge's row offsets are loaded in one round.
+    // registers.  Every edge issues its loads whether it is cut or not (a
+    // non-cut edge reads row 0 of slot 0, which always exists) and nothing
+    // loaded is used before all are issued, so the offsets of all cut rows
+    // arrive in one round trip instead of one per edge.
     template <class C, class A>
     TFSM_LANE_FN void enter_adaptive(C &c, const A &a, uint32_t dd, uint32_t w, uint32_t prim) {
-        uint32_t lo[kCutMax], ln[kCutMax];
+        uint32_t sel[kCutMax], rr[kCutMax];
         uint32_t cut = 0, dlo = 0, dhi = 0;     // cut edges; their unmatched ends, 4 bits each
 #pragma unroll
         for (uint32_t e = 0; e < kCutMax; ++e) {
-            lo[e] = 0;
-            ln[e] = 0xffffffffu;
             const uint32_t xy = (ep[e >> 2] >> ((e & 3u) * 8u)) & 0xffu;
-            if (xy == 0xffu) continue;
-            const uint32_t x = xy & 15u, y = xy >> 4;
+            const bool edge = xy != 0xffu;
+            const uint32_t x = edge ? xy & 15u : 0u, y = edge ? xy >> 4 : 0u;
             const uint32_t mx = (vis >> x) & 1u, my = (vis >> y) & 1u;
             const uint32_t dst = mx ? y : x;
-            if (mx != my && (w == kNone || dst == w)) {
-                uint32_t l, h;
-                a.edge_row(e, mx != 0, c.idx(mx ? x : y), l, h);
-                lo[e] = l;
-                ln[e] = h - l;
-                cut |= 1u << e;
-                if (e < 8) dlo |= dst << (4u * e);
-                else dhi |= dst << (4u * (e - 8));
-            }
+            const bool is_cut = edge && mx != my && (w == kNone || dst == w);
+            sel[e] = is_cut ? 2u * e + (mx ? 0u : 1u) : 0u;
+            rr[e] = c.idx(is_cut ? (mx ? x : y) : 0u);
+            if (!is_cut) rr[e] = 0;
+            cut |= (is_cut ? 1u : 0u) << e;
+            if (e < 8) dlo |= (is_cut ? dst : 0u) << (4u * e);
+            else dhi |= (is_cut ? dst : 0u) << (4u * (e - 8));
         }
+        uint32_t ao[kCutMax];
+#pragma unroll
+        for (uint32_t e = 0; e < kCutMax; ++e) ao[e] = a.cut_ao(sel[e]);
+        uint32_t lo[kCutMax], hi[kCutMax];
+#pragma unroll
+        for (uint32_t e = 0; e < kCutMax; ++e) a.row_at(ao[e], rr[e], lo[e], hi[e]);
         auto dst_of = [&](uint32_t e) { return e < 8 ? (dlo >> (4u * e)) & 15u : (dhi >> (4u * (e - 8))) & 15u; };
+        auto len_of = [&](uint32_t e) { return ((cut >> e) & 1u) ? hi[e] - lo[e] : 0xffffffffu; };
         if (w == kNone) {
             uint32_t best = 0xffffffffu;
 #pragma unroll
-            for (uint32_t e = 0; e < kCutMax; ++e)
-                if (((cut >> e) & 1u) && ln[e] < best) {
-                    best = ln[e];
+            for (uint32_t e = 0; e < kCutMax; ++e) {
+                const uint32_t l = len_of(e);
+                if (((cut >> e) & 1u) && l < best) {
+                    best = l;
                     w = dst_of(e);
                 }
+            }
         }
         uint32_t nb = 0, pick = 0, plen = 0xffffffffu;
 #pragma unroll
         for (uint32_t e = 0; e < kCutMax; ++e)
             if (((cut >> e) & 1u) && dst_of(e) == w) {
-                if (ln[e] < plen) {
-                    plen = ln[e];
+                if (len_of(e) < plen) {
+                    plen = len_of(e);
                     pick = nb;
                 }
                 ++nb;
@@ -220,11 +230,11 @@ struct Lane {
             if (((cut >> e) & 1u) && dst_of(e) == w) {
                 if (k == prim) {
                     plo = lo[e];
-                    phi = lo[e] + ln[e];
+                    phi = hi[e];
                 } else {
                     if (g < MV) {
                         c.cur(g) = lo[e];
-                        c.cend(g) = lo[e] + ln[e];
+                        c.cend(g) = hi[e];
                         c.curv(g) = kUnknown;
                     }
                     ++g;

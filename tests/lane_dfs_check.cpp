@@ -152,8 +152,14 @@ struct HostAcc {
         }
         return w;
     }
-    void edge_row(uint32_t e, bool from_x, uint32_t r, uint32_t &lo, uint32_t &hi) const {
-        row(from_x ? edge_x(e) * n + edge_y(e) : edge_y(e) * n + edge_x(e), r, lo, hi);
+    // the host keeps the slot itself as its "offset position"
+    uint32_t cut_ao(uint32_t slot) const {
+        if (slot >= 2 * edges.size()) violation("cut_ao", slot, 2 * edges.size());
+        return slot;
+    }
+    void row_at(uint32_t slot, uint32_t r, uint32_t &lo, uint32_t &hi) const {
+        const uint32_t e = slot / 2;
+        row(slot % 2 == 0 ? edge_x(e) * n + edge_y(e) : edge_y(e) * n + edge_x(e), r, lo, hi);
     }
     uint32_t same_lab(uint32_t x) const {
         if (x >= n) violation("same_lab", x, n);
