@@ -136,7 +136,7 @@ typedef struct tfsm_ctx tfsm_ctx;
 int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out);
 int tfsm_ctx_destroy(tfsm_ctx *ctx);
 /* Capacity of the device pool that holds the auxiliary tables (entries of 4
- * bytes; the default is max(4 x adjacency entries, 2^22)).  It still grows on
+ * bytes; the default is max(8 x adjacency entries, 2^22)).  It still grows on
  * demand: a call whose tables do not fit is redone with the pool enlarged.
  * No counterpart in the reference, which allocates its tables per pattern
  * (gmatch.h:995-1130). */
@@ -180,14 +180,14 @@ int64_t tfsm_read_trace(tfsm_ctx *ctx, uint32_t *out, uint64_t cap);
  * library's stream.  Algorithmic bytes count each input byte the phase must
  * read and each output byte it must write once:
  *   table  : per directed pattern edge a->b, 8 B of segment bounds and 4 B per
- *            scanned neighbour id for every v in C(a), plus 4 B per row offset
- *            and per table entry written;
+ *            scanned neighbour id for every v in C(a), plus 8 B per row
+ *            (start, end) and 4 B per table entry written;
  *   search : per (u, v) verdict, sum over pattern neighbours u' of u of
  *            8 B of row bounds + 4 B per entry of the depth-1 row A[u->u'][v]
  *            (taken at the mean row length of u). */
 typedef struct {
     float filter_ms;            /* LDF + DP-iso passes + compaction (incl. host round trips) */
-    float table_ms;             /* count + scan + fill */
+    float table_ms;             /* one-pass table build */
     float search_ms;            /* existence-search kernel launch alone */
     uint32_t search_launches;
     uint64_t table_bytes;

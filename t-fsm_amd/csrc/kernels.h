@@ -66,7 +66,7 @@ struct PatStatic {
     // directed pattern edges
     uint8_t slot_a[kMaxSlots], slot_b[kMaxSlots];
     uint32_t slot_key[kMaxSlots];
-    uint32_t *aux_off[kMaxSlots];       // |C(a)| + 1 relative offsets
+    uint32_t *aux_off[kMaxSlots];       // per row of C(a): (start, end) absolute pool positions
     // matching plans per start vertex u: order[u][d], bn_slot[u][d][k]
     uint8_t order[kMaxV][kMaxV];
     uint8_t bn_n[kMaxV][kMaxV];
@@ -80,7 +80,7 @@ struct PatStatic {
     uint32_t adaptive;
     uint16_t same_lab[kMaxV];
     uint32_t edge_pack[kCutEdges / 4];  // edges a | b << 4, one byte each, 0xff past the last
-    uint32_t cut_ao[2 * kCutEdges];     // slot's row offsets at aux_region + cut_ao[slot]
+    uint32_t cut_ao[2 * kCutEdges];     // slot's row pairs at aux_region + cut_ao[slot]
 };
 
 // Device-written state of one pattern.
@@ -91,8 +91,14 @@ struct PatState {
     uint32_t ndone;                     // vertices that reached freq valid candidates
     uint32_t status;                    // 0 undecided, 1 frequent, 2 infrequent
     unsigned long long searches;
-    uint32_t aux_base[kMaxSlots];       // table start in the pool
-    uint32_t aux_total[kMaxSlots];
+    uint32_t aux_total[kMaxSlots];      // entries of each slot's table
+};
+
+// One device-to-device copy of `words` 32-bit words (a zero fill when src is null).
+struct CopyDesc {
+    const uint32_t *src;
+    uint32_t *dst;
+    unsigned long long words;
 };
 
 // Flattened work over (pattern, x, item) ranges: item t belongs to range m

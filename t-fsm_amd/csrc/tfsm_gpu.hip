@@ -209,10 +209,10 @@ struct tfsm_ctx {
     // last call
     std::vector<PatStatic> last_static;
     std::vector<PatState> last_state;
+    uint32_t last_pool_used = 0;            // pool entries the last call's tables reserved
     std::vector<int> last_decided;
     uint32_t *d_trace = nullptr;
     uint64_t trace_len = 0, trace_cap = 0;
-    unsigned long long *d_scanned = nullptr;
     // ranker
     uint16_t *d_rank_w = nullptr;            // installed BF16 parameters (kParams)
     bool ranker_installed = false;
@@ -266,11 +266,13 @@ T *upload(tfsm_ctx *c, const std::vector<T> &v) {
 struct HostWork {
     std::vector<unsigned long long> base{0};
     std::vector<uint32_t> pat, x;
-    void add(uint32_t p, uint32_t xx, uint64_t count) {
+    // `align`: the range takes a multiple of that many items (the excess
+    // items fall outside the range's own count, which kernels read)
+    void add(uint32_t p, uint32_t xx, uint64_t count, uint64_t align = 1) {
         if (count == 0) return;
         pat.push_back(p);
         x.push_back(xx);
-        base.push_back(base.back() + count);
+        base.push_back(base.back() + (count + align - 1) / align * align);
     }
     WorkTable to_device(tfsm_ctx *c) const {
         WorkTable w;
@@ -564,13 +566,12 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             P.task_prefix[u + 1] = P.task_prefix[u] + P.dom_size[u];
             verdict_words += 2 * (size_t)((P.dom_size[u] + 31) / 32);
         }
-        for (uint32_t sl = 0; sl < P.nslots; ++sl) aux_words += P.dom_size[P.slot_a[sl]] + 1;
+        for (uint32_t sl = 0; sl < P.nslots; ++sl) aux_words += 2 * (size_t)P.dom_size[P.slot_a[sl]];
     }
     uint32_t *aux_region = c->work.alloc<uint32_t>(std::max<size_t>(aux_words, 1));
     uint32_t *verdict_region = c->work.alloc<uint32_t>(std::max<size_t>(verdict_words, 1));
     if (verdict_words) HIPX(hipMemsetAsync(verdict_region, 0, verdict_words * sizeof(uint32_t), s));
     HostWork aux_work;
-    std::vector<uint32_t> slot_p, slot_s;
     std::vector<unsigned long long> task_base{0};
     {
         size_t ao = 0, vo = 0;
@@ -581,10 +582,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                 for (uint32_t sl = 0; sl < P.nslots; ++sl) {
                     if (sl < 2u * kCutEdges) P.cut_ao[sl] = (uint32_t)ao;
                     P.aux_off[sl] = aux_region + ao;
-                    ao += P.dom_size[P.slot_a[sl]] + 1;
-                    aux_work.add(p, sl, P.dom_size[P.slot_a[sl]]);
-                    slot_p.push_back(p);
-                    slot_s.push_back(sl);
+                    ao += 2 * (size_t)P.dom_size[P.slot_a[sl]];
+                    aux_work.add(p, sl, P.dom_size[P.slot_a[sl]], 64);   // one slot per wave
                 }
                 for (uint32_t u = 0; u < P.nv; ++u) {
                     const size_t wds = (P.dom_size[u] + 31) / 32;
@@ -599,28 +598,16 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     stage_copy(c, d_ps, hs.data(), n * sizeof(PatStatic));
 
     // ---------------- auxiliary tables ----------------
-    if (!slot_p.empty()) {
+    if (aux_work.base.back()) {
         WorkTable waux = aux_work.to_device(c);
-        const uint32_t *d_slot_p = upload(c, slot_p), *d_slot_s = upload(c, slot_s);
         const uint32_t gaux = grid_for(waux.total, c->num_cus);
         // no host round trip: an overflow of the pool is found with the
         // final results (the search kernels do nothing on an overflowed
         // pool), and the call is then redone with a larger pool
-        HIPX(hipMemsetAsync(c->d_scanned, 0, sizeof(unsigned long long), s));
-        if (waux.total)
-            hipLaunchKernelGGL(k_aux_count, dim3(gaux), dim3(kBlock), 0, s, c->g, d_ps, waux, c->d_scanned);
-        hipLaunchKernelGGL(k_aux_scan, dim3((uint32_t)slot_p.size()), dim3(kBlock), 0, s, d_ps, d_st, d_slot_p,
-                           d_slot_s, c->d_flags, c->pool_cap, c->d_flags + 1);
-        if (waux.total)
-            hipLaunchKernelGGL(k_aux_fill, dim3(gaux), dim3(kBlock), 0, s, c->g, d_ps, d_st, waux, c->d_pool,
-                               c->d_flags + 1);
+        hipLaunchKernelGGL(k_aux_build, dim3(gaux), dim3(kBlock), 0, s, c->g, d_ps, d_st, waux, c->d_pool,
+                           c->d_flags, c->pool_cap, c->d_flags + 1);
         HIPX(hipGetLastError());
     }
-    // neighbour ids the table build scanned, read with the final results
-    unsigned long long *h_scanned = reinterpret_cast<unsigned long long *>(c->stage.alloc(sizeof(unsigned long long)));
-    *h_scanned = 0;
-    if (!slot_p.empty())
-        HIPX(hipMemcpyAsync(h_scanned, c->d_scanned, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPX(hipEventRecord(c->ev[2], s));
 
     // ---------------- existence search ----------------
@@ -815,14 +802,15 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     }
     {
         // algorithmic bytes (definitions in tfsm_gpu.h)
-        unsigned long long tb = 4ull * *h_scanned, sb = 0;
+        // the pool reservation is the neighbour ids the table build scanned
+        unsigned long long tb = 4ull * hpool[0], sb = 0;
         for (uint32_t p = 0; p < n; ++p) {
             if (decided[p] != TFSM_DECIDED_SEARCH) continue;
             const PatStatic &P = hs[p];
             const PatState &S = hst[p];
             for (uint32_t sl = 0; sl < P.nslots; ++sl) {
                 const unsigned long long rows = P.dom_size[P.slot_a[sl]];
-                tb += 8ull * rows + 4ull * (rows + 1) + 4ull * S.aux_total[sl];
+                tb += 16ull * rows + 4ull * S.aux_total[sl];
             }
             for (uint32_t u = 0; u < P.nv; ++u) {
                 if (!P.dom_size[u]) continue;
@@ -837,6 +825,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     }
 
     // ---------------- results and retained child domains ----------------
+    std::vector<CopyDesc> keep;                 // the retained domains' copies, one launch
     for (uint32_t p = 0; p < n; ++p) {
         const PatStatic &P = hs[p];
         const PatState &S = hst[p];
@@ -875,21 +864,22 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                 R.size[u] = P.dom_size[u];
                 R.dom[u] = b; b += P.dom_size[u];
                 R.invalid[u] = b; b += wds;
-                if (P.dom_size[u])
-                    HIPX(hipMemcpyAsync(R.dom[u], P.dom[u], P.dom_size[u] * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-                if (wds) {
-                    if (decided[p] == TFSM_DECIDED_SEARCH)
-                        HIPX(hipMemcpyAsync(R.invalid[u], P.invalid_bits[u], wds * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-                    else
-                        HIPX(hipMemsetAsync(R.invalid[u], 0, wds * sizeof(uint32_t), s));
-                }
+                if (P.dom_size[u]) keep.push_back({P.dom[u], R.dom[u], P.dom_size[u]});
+                if (wds) keep.push_back({decided[p] == TFSM_DECIDED_SEARCH ? P.invalid_bits[u] : nullptr, R.invalid[u], wds});
             }
             r.domain_handle = c->next_handle++;
             c->retained.emplace(r.domain_handle, R);
         }
     }
+    if (!keep.empty()) {
+        const CopyDesc *d_keep = upload(c, keep);
+        hipLaunchKernelGGL(k_copy_batch, dim3((uint32_t)std::min<size_t>(keep.size(), 4096)), dim3(kBlock), 0, s,
+                           d_keep, (uint32_t)keep.size());
+        HIPX(hipGetLastError());
+    }
     c->last_static = hs;
     c->last_state.assign(hst, hst + n);
+    c->last_pool_used = hpool[0];
     c->last_decided = decided;
     HIPX(hipStreamSynchronize(s));
     return TFSM_OK;
@@ -997,7 +987,7 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         c->g.EL = c->EL;
         c->g.seg = c->d_seg;
         c->g.nbr = c->d_nbr;
-        c->pool_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(4 * g->num_adj, 1u << 22), 0xfffffff0ull);
+        c->pool_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * g->num_adj, 1u << 22), 0xfffffff0ull);
         HIPX(hipMalloc(&c->d_pool, (size_t)c->pool_cap * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_flags, 16 * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_pflags, kMaxPrunePasses * sizeof(uint32_t)));
@@ -1007,7 +997,6 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         HIPX(hipMalloc(&c->d_piece_inc, (size_t)kSplitCap * sizeof(uint32_t)));
         HIPX(hipMemset(c->d_records, 0, (size_t)kSplitCap * sizeof(SplitRecord)));
         HIPX(hipMemset(c->d_piece_inc, 0, (size_t)kSplitCap * sizeof(uint32_t)));
-        HIPX(hipMalloc(&c->d_scanned, sizeof(unsigned long long)));
         HIPX(hipMalloc(&c->d_rank_w, kParams * sizeof(uint16_t)));
         return TFSM_OK;
     });
@@ -1046,7 +1035,6 @@ int tfsm_ctx_destroy(tfsm_ctx *c) {
     (void)hipFree(c->d_records);
     (void)hipFree(c->d_piece_cnt);
     (void)hipFree(c->d_piece_inc);
-    (void)hipFree(c->d_scanned);
     (void)hipFree(c->d_pflags);
     (void)hipFree(c->d_rank_w);
     (void)hipFree(c->d_feat);
@@ -1146,16 +1134,24 @@ int64_t tfsm_last_aux(tfsm_ctx *ctx, uint32_t a, uint32_t b, uint32_t *off, uint
     const uint32_t rows = P.dom_size[a];
     const uint32_t total = ctx->last_state[0].aux_total[slot];
     int rc = guarded(ctx, [&]() -> int {
-        if (off) {
-            // the device keeps absolute pool positions; callers get offsets within the table
-            const uint32_t cnt = std::min(rows + 1, off_cap);
-            HIPX(hipMemcpy(off, P.aux_off[slot], cnt * sizeof(uint32_t), hipMemcpyDeviceToHost));
-            const uint32_t base = ctx->last_state[0].aux_base[slot];
-            for (uint32_t i = 0; i < cnt; ++i) off[i] -= base;
+        // the device keeps each row as a (start, end) pair of pool positions;
+        // callers get the table as a plain CSR
+        std::vector<uint32_t> se(2 * (size_t)rows), pool(idx ? ctx->last_pool_used : 0u);
+        if (rows) HIPX(hipMemcpy(se.data(), P.aux_off[slot], se.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (idx && total && !pool.empty())
+            HIPX(hipMemcpy(pool.data(), ctx->d_pool, pool.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        uint32_t acc = 0;
+        for (uint32_t i = 0; i <= rows; ++i) {
+            if (off && i < off_cap) off[i] = acc;
+            if (i == rows) break;
+            const uint32_t lo = se[2 * i], hi = std::max(lo, se[2 * i + 1]);
+            if (!idx) {
+                acc += hi - lo;
+                continue;
+            }
+            for (uint32_t q = lo; q < hi; ++q, ++acc)
+                if (acc < idx_cap) idx[acc] = q < pool.size() ? pool[q] : 0xffffffffu;
         }
-        if (idx && total)
-            HIPX(hipMemcpy(idx, ctx->d_pool + ctx->last_state[0].aux_base[slot], std::min(total, idx_cap) * sizeof(uint32_t),
-                           hipMemcpyDeviceToHost));
         return TFSM_OK;
     });
     return rc == TFSM_OK ? (int64_t)total : rc;
