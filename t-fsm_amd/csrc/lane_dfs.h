@@ -49,6 +49,10 @@ constexpr uint32_t kUnknown = 0xffffffffu;   // curv: entry under the cursor not
 
 enum LaneStep : uint32_t { kLaneRunning = 0, kLaneFound = 1, kLaneExhausted = 2 };
 
+// The matching orders a search instance handles: either, per pattern at run
+// time, or only one (the other's code is then not compiled in).
+enum LaneOrder : int { kOrderMixed = 0, kOrderAdaptive = 1, kOrderReference = 2 };
+
 // Depth metadata, 16 bits: the primary row among the depth's backward rows
 // (bits 0-3), the depth's pattern vertex (4-7; adaptive order), the first
 // cursor slot of its other rows (8-11) and their number (12-15).
@@ -109,11 +113,12 @@ TFSM_LANE_FN uint32_t lane_gallop(const A &a, uint32_t lo, uint32_t hi, uint32_t
 // label).
 // C: idx(x), vid(j), pos(d), end(d), cur(g), cend(g), curv(g), meta(d), all
 // lvalues.
-template <int MV>
+template <int MV, int ORD = kOrderMixed>
 struct Lane {
     uint32_t u = 0, n = 0, d = 0, floor = 1;
     uint32_t vis = 0;              // adaptive order: pattern vertices of depths 0 .. d
-    bool adaptive = false;
+    bool adaptive_rt = false;      // the pattern's order (kOrderMixed)
+    TFSM_LANE_FN bool adaptive() const { return ORD == kOrderAdaptive || (ORD == kOrderMixed && adaptive_rt); }
     uint32_t pf_pos = kUnknown, pf_val = 0;   // prefetched pool entry (the pool is read-only)
     uint32_t ep[kCutMax / 4] = {};            // adaptive order: the pattern's packed edges
 
@@ -251,11 +256,11 @@ struct Lane {
     TFSM_LANE_FN void start_root(C &c, const A &a, uint32_t u_, uint32_t ci) {
         u = u_;
         n = a.nv();
-        adaptive = a.adaptive();
+        adaptive_rt = a.adaptive();
         floor = 1;
         d = 1;
         c.idx(u) = ci;
-        if (adaptive) {
+        if (adaptive()) {
             load_edges(a);
             vis = 1u << u;
             c.meta(0) = meta_pack(0, u, 0, 0);
@@ -275,10 +280,10 @@ struct Lane {
                                   const uint8_t *pord, uint32_t prim, uint32_t lo, uint32_t hi) {
         u = u_;
         n = a.nv();
-        adaptive = a.adaptive();
+        adaptive_rt = a.adaptive();
         floor = floor_;
         d = floor_;
-        if (adaptive) {
+        if (adaptive()) {
             load_edges(a);
             vis = 0;
             for (uint32_t j = 0; j < floor_; ++j) {
@@ -304,7 +309,7 @@ struct Lane {
     // Pattern vertex of depth j (j <= d).
     template <class C, class A>
     TFSM_LANE_FN uint32_t vertex_at(C &c, const A &a, uint32_t j) const {
-        return adaptive ? meta_vertex(c.meta(j)) : a.order(u, j);
+        return adaptive() ? meta_vertex(c.meta(j)) : a.order(u, j);
     }
 
     // Shallowest depth >= floor with untried candidates (d + 1 if none).
@@ -321,7 +326,7 @@ struct Lane {
     TFSM_LANE_FN uint32_t step(C &c, const A &a) {
         const uint32_t pos = c.pos(d), end = c.end(d);
         if (pos >= end) {
-            if (adaptive) vis &= ~(1u << meta_vertex(c.meta(d)));
+            if (adaptive()) vis &= ~(1u << meta_vertex(c.meta(d)));
             --d;
             return d < floor ? kLaneExhausted : kLaneRunning;
         }
@@ -329,7 +334,7 @@ struct Lane {
         const uint32_t m = c.meta(d);
         const uint32_t prim = meta_prim(m);
         uint32_t w, nb, g0, sl;
-        if (adaptive) {
+        if (adaptive()) {
             w = meta_vertex(m);
             nb = meta_secn(m) + 1;
             g0 = meta_secb(m);
@@ -352,7 +357,8 @@ struct Lane {
             if (k == prim) continue;
             const uint32_t g = g0 + j++;
             bool member;
-            if (g < MV) {
+            // the adaptive order's rows always have cursor slots
+            if (ORD == kOrderAdaptive || g < MV) {
                 const uint32_t cur = c.cur(g), hi = c.cend(g);
                 if (cur >= hi) return kLaneRunning;
                 uint32_t cv = c.curv(g), at = cur;
@@ -372,7 +378,7 @@ struct Lane {
             if (!member) return kLaneRunning;
         }
         if (sl) {
-            if (adaptive) {
+            if (adaptive()) {
                 for (uint32_t q = sl & vis & ~(1u << w); q; q &= q - 1)
                     if (c.vid(lane_ctz(q)) == v) return kLaneRunning;
                 c.vid(w) = v;
@@ -385,7 +391,7 @@ struct Lane {
         c.idx(w) = cand;
         if (d + 1 == n) return kLaneFound;
         ++d;
-        if (adaptive) enter_adaptive(c, a, d, kNone, kNone);
+        if (adaptive()) enter_adaptive(c, a, d, kNone, kNone);
         else enter_reference(c, a, d, kNone);
         return kLaneRunning;
     }

@@ -180,7 +180,7 @@ constexpr uint32_t kPruneBatch = 6;          // passes launched between converge
 struct tfsm_ctx {
     int device = 0;
     int num_cus = 0;
-    int verify_blocks8 = 1, verify_blocks16 = 1;    // resident search workgroups per CU
+    int verify_blocks8 = 1, verify_blocks8a = 1, verify_blocks16 = 1;   // resident search workgroups per CU
     hipStream_t stream = nullptr;
     // graph
     uint32_t n = 0, L = 0, EL = 0, K = 0, W = 0;
@@ -716,10 +716,16 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         HIPX(hipMemsetAsync(c->d_ctr, 0, sizeof(SearchCounters), s));
         HIPX(hipEventRecord(c->ev[4], s));
         uint32_t max_nv = 0;
-        for (uint32_t p = 0; p < n; ++p) if (decided[p] == TFSM_DECIDED_SEARCH) max_nv = std::max(max_nv, hs[p].nv);
+        bool all_adaptive = true;       // the instance without the reference order's code
+        for (uint32_t p = 0; p < n; ++p)
+            if (decided[p] == TFSM_DECIDED_SEARCH) {
+                max_nv = std::max(max_nv, hs[p].nv);
+                all_adaptive = all_adaptive && hs[p].adaptive;
+            }
         // persistent grid: at most the resident blocks per CU (registers and
         // LDS: 52 KB per block for the 8-deep variant, 104 KB for the 16-deep one)
-        const uint32_t resident = (uint32_t)(max_nv <= 8 ? c->verify_blocks8 : c->verify_blocks16);
+        const uint32_t resident =
+            (uint32_t)(max_nv > 8 ? c->verify_blocks16 : all_adaptive ? c->verify_blocks8a : c->verify_blocks8);
         const uint32_t per_cu = opt.grid_waves_per_cu
                                     ? std::min(resident, std::max(1u, opt.grid_waves_per_cu / (kBlock / 64)))
                                     : resident;
@@ -743,8 +749,12 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         // to decide the patterns, then the deferred rest (mostly skipped)
         const bool two_parts = va.order && va.split && !va.long_cap;
         auto launch_verify = [&]() {
-            if (max_nv <= 8) hipLaunchKernelGGL(k_verify<8>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
-            else hipLaunchKernelGGL(k_verify<16>, dim3(grid), dim3(kBlock), 0, s, va, epoch);
+            if (max_nv > 8)
+                hipLaunchKernelGGL((k_verify<16, tfsm_lane::kOrderMixed>), dim3(grid), dim3(kBlock), 0, s, va, epoch);
+            else if (all_adaptive)
+                hipLaunchKernelGGL((k_verify<8, tfsm_lane::kOrderAdaptive>), dim3(grid), dim3(kBlock), 0, s, va, epoch);
+            else
+                hipLaunchKernelGGL((k_verify<8, tfsm_lane::kOrderMixed>), dim3(grid), dim3(kBlock), 0, s, va, epoch);
             HIPX(hipGetLastError());
         };
         va.part = two_parts ? 1u : 0u;
@@ -924,9 +934,14 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         c->num_cus = prop.multiProcessorCount;
         // the persistent search grid is sized to what stays resident
         // (registers and LDS of each instance)
-        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->verify_blocks8, k_verify<8>, kBlock, 0));
-        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->verify_blocks16, k_verify<16>, kBlock, 0));
+        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->verify_blocks8, k_verify<8, tfsm_lane::kOrderMixed>,
+                                                          kBlock, 0));
+        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->verify_blocks8a, k_verify<8, tfsm_lane::kOrderAdaptive>,
+                                                          kBlock, 0));
+        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->verify_blocks16, k_verify<16, tfsm_lane::kOrderMixed>,
+                                                          kBlock, 0));
         c->verify_blocks8 = std::max(1, c->verify_blocks8);
+        c->verify_blocks8a = std::max(1, c->verify_blocks8a);
         c->verify_blocks16 = std::max(1, c->verify_blocks16);
         HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto &e : c->ev) HIPX(hipEventCreate(&e));

@@ -186,14 +186,17 @@ struct TaskResult {
 };
 
 // Runs task (u, ci) with pieces; `seq` (when set) collects the entered
-// (depth, candidate) pairs of the root piece.
-TaskResult run_task(const HostAcc &a, uint32_t u, uint32_t ci, uint32_t donate_every, uint32_t fanout,
-                    const std::vector<std::pair<uint32_t, uint32_t>> &edges_pat,
-                    std::vector<std::pair<uint32_t, uint32_t>> *seq) {
+// (depth, candidate) pairs of the root piece.  ORD: the lane instance (the
+// adaptive-only one is what the search kernel runs when every pattern of a
+// launch is in the adaptive order).
+template <int ORD>
+TaskResult run_task_in(const HostAcc &a, uint32_t u, uint32_t ci, uint32_t donate_every, uint32_t fanout,
+                       const std::vector<std::pair<uint32_t, uint32_t>> &edges_pat,
+                       std::vector<std::pair<uint32_t, uint32_t>> *seq) {
     TaskResult res;
     HostCols c;
     c.n = a.n;
-    tfsm_lane::Lane<kMV> L;
+    tfsm_lane::Lane<kMV, ORD> L;
     std::vector<Piece> queue;
     auto check_embedding = [&]() {
         if (c.idx(u) != ci) violation("embedding root", c.idx(u), ci);
@@ -266,6 +269,15 @@ TaskResult run_task(const HostAcc &a, uint32_t u, uint32_t ci, uint32_t donate_e
         ++res.pieces;
     }
     return res;
+}
+
+// Adaptive-order tasks alternate between the two instances that run them.
+TaskResult run_task(const HostAcc &a, uint32_t u, uint32_t ci, uint32_t donate_every, uint32_t fanout,
+                    const std::vector<std::pair<uint32_t, uint32_t>> &edges_pat,
+                    std::vector<std::pair<uint32_t, uint32_t>> *seq) {
+    if (a.adaptive_ && (ci & 1u))
+        return run_task_in<tfsm_lane::kOrderAdaptive>(a, u, ci, donate_every, fanout, edges_pat, seq);
+    return run_task_in<tfsm_lane::kOrderMixed>(a, u, ci, donate_every, fanout, edges_pat, seq);
 }
 
 // Random instances: vertices 0..n-1 with labels in {0, 1}; a connected
