@@ -263,6 +263,61 @@ T *upload(tfsm_ctx *c, const std::vector<T> &v) {
     return d;
 }
 
+// One phase's uploads sent with one copy and its zero fills done in one
+// launch.  The device block mirrors a pinned staging block byte for byte, so
+// an upload's device address is known when it is added; an upload beyond the
+// block's capacity goes on its own.  Fills are split into pieces of at most
+// kFillPiece words, one workgroup each.
+constexpr unsigned long long kFillPiece = 1u << 14;
+
+struct PhaseBatch {
+    tfsm_ctx *c;
+    char *host, *dev;
+    size_t cap, used = 0;
+    std::vector<CopyDesc> zeros;
+    PhaseBatch(tfsm_ctx *c_, size_t cap_);
+    template <class T>
+    T *add(const T *h, size_t count) {
+        const size_t bytes = (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
+        if (used + bytes > cap) {
+            T *d = c->work.alloc<T>(std::max<size_t>(count, 1));
+            if (count) stage_copy(c, d, h, count * sizeof(T));
+            return d;
+        }
+        T *d = reinterpret_cast<T *>(dev + used);
+        if (count) std::memcpy(host + used, h, count * sizeof(T));
+        used += bytes;
+        return d;
+    }
+    template <class T>
+    T *add(const std::vector<T> &v) {
+        return add(v.data(), v.size());
+    }
+    void zero(void *d, size_t bytes) {       // bytes: a multiple of 4
+        uint32_t *w = static_cast<uint32_t *>(d);
+        for (unsigned long long o = 0, n = bytes / 4; o < n; o += kFillPiece)
+            zeros.push_back({nullptr, w + o, std::min(kFillPiece, n - o)});
+    }
+    void flush();
+};
+
+PhaseBatch::PhaseBatch(tfsm_ctx *c_, size_t cap_) : c(c_), cap((cap_ + 255) & ~(size_t)255) {
+    host = static_cast<char *>(c->stage.alloc(std::max<size_t>(cap, 256)));
+    dev = c->work.alloc<char>(std::max<size_t>(cap, 256));
+}
+
+void PhaseBatch::flush() {
+    const CopyDesc *dz = zeros.empty() ? nullptr : add(zeros);
+    if (used) HIPX(hipMemcpyAsync(dev, host, used, hipMemcpyHostToDevice, c->stream));
+    if (dz) {
+        hipLaunchKernelGGL(k_copy_batch, dim3((uint32_t)std::min<size_t>(zeros.size(), 4096)), dim3(kBlock), 0,
+                           c->stream, dz, (uint32_t)zeros.size());
+        HIPX(hipGetLastError());
+    }
+    used = cap;                 // later uploads go on their own
+    zeros.clear();
+}
+
 struct HostWork {
     std::vector<unsigned long long> base{0};
     std::vector<uint32_t> pat, x;
@@ -274,15 +329,17 @@ struct HostWork {
         x.push_back(xx);
         base.push_back(base.back() + (count + align - 1) / align * align);
     }
-    WorkTable to_device(tfsm_ctx *c) const {
+    WorkTable to_device(PhaseBatch &b) const {
         WorkTable w;
-        w.base = upload(c, base);
-        w.pat = upload(c, pat);
-        w.x = upload(c, x);
+        w.base = b.add(base);
+        w.pat = b.add(pat);
+        w.x = b.add(x);
         w.M = (uint32_t)pat.size();
         w.total = base.back();
         return w;
     }
+    // device bytes of to_device, with the rounding of the batch
+    size_t bytes() const { return 3 * 256 + base.size() * 8 + 2 * pat.size() * 4; }
 };
 
 // Pattern checks: sizes, labels, edge endpoints, no duplicate edges, connected.
@@ -491,7 +548,6 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     // one region for all bitmaps (zeroed at once) and one for all lists
     uint32_t *bits_region = c->work.alloc<uint32_t>(bits_words);
     uint32_t *dom_region = c->work.alloc<uint32_t>(dom_words);
-    HIPX(hipMemsetAsync(bits_region, 0, bits_words * sizeof(uint32_t), s));
     {
         size_t bo = 0, dofs = 0;
         for (uint32_t p = 0; p < n; ++p)
@@ -501,10 +557,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                 hs[p].dom[u] = dom_region + dofs; dofs += std::max<uint32_t>(hs[p].src_n[u], 1);
             }
     }
-    PatStatic *d_ps = c->work.alloc<PatStatic>(n);
     PatState *d_st = c->work.alloc<PatState>(n);
-    stage_copy(c, d_ps, hs.data(), n * sizeof(PatStatic));
-    HIPX(hipMemsetAsync(d_st, 0, n * sizeof(PatState), s));
 
     // ---------------- filter: LDF, DP-iso to the fixpoint, compaction ----------------
     HostWork src_work;
@@ -515,8 +568,17 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             pair_p.push_back(p);
             pair_u.push_back(u);
         }
-    WorkTable wsrc = src_work.to_device(c);
-    const uint32_t *d_pair_p = upload(c, pair_p), *d_pair_u = upload(c, pair_u);
+    // this phase's uploads in one copy, its fills in one launch
+    PhaseBatch b1(c, n * sizeof(PatStatic) + src_work.bytes() + 2 * 256 + 8 * pair_p.size() +
+                         sizeof(CopyDesc) * (bits_words / kFillPiece + n * sizeof(PatState) / 4 / kFillPiece + 8) + 512);
+    PatStatic *d_ps = b1.add(hs.data(), n);
+    WorkTable wsrc = src_work.to_device(b1);
+    const uint32_t *d_pair_p = b1.add(pair_p), *d_pair_u = b1.add(pair_u);
+    b1.zero(bits_region, bits_words * sizeof(uint32_t));
+    b1.zero(d_st, n * sizeof(PatState));
+    b1.zero(c->d_flags, 2 * sizeof(uint32_t));        // pool used, pool overflow
+    if (wsrc.total) b1.zero(c->d_pflags, kMaxPrunePasses * sizeof(uint32_t));
+    b1.flush();
     const uint32_t gsrc = grid_for(wsrc.total, c->num_cus);
     if (wsrc.total) hipLaunchKernelGGL(k_ldf, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc);
     HIPX(hipGetLastError());
@@ -527,8 +589,6 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     // another batch is needed (compaction is then simply redone)
     PatState *hst = c->h_state.ensure(n);
     uint32_t passes = 0;
-    if (wsrc.total) HIPX(hipMemsetAsync(c->d_pflags, 0, kMaxPrunePasses * sizeof(uint32_t), s));
-    HIPX(hipMemsetAsync(c->d_flags, 0, 2 * sizeof(uint32_t), s));     // pool used, pool overflow
     for (;;) {
         for (uint32_t i = 0; wsrc.total && i < kPruneBatch; ++i, ++passes) {
             if (passes >= kMaxPrunePasses) throw ArgError{TFSM_ERR_LIMIT, "filter did not converge"};
@@ -570,7 +630,6 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     }
     uint32_t *aux_region = c->work.alloc<uint32_t>(std::max<size_t>(aux_words, 1));
     uint32_t *verdict_region = c->work.alloc<uint32_t>(std::max<size_t>(verdict_words, 1));
-    if (verdict_words) HIPX(hipMemsetAsync(verdict_region, 0, verdict_words * sizeof(uint32_t), s));
     HostWork aux_work;
     std::vector<unsigned long long> task_base{0};
     {
@@ -595,11 +654,19 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             task_base.push_back(task_base.back() + ntask);
         }
     }
-    stage_copy(c, d_ps, hs.data(), n * sizeof(PatStatic));
+    // this phase's uploads (the descriptors with their plans, the table
+    // work, the task bases) in one copy, the verdict bitmaps' fill in one
+    // launch; the later kernels take the new descriptor copy
+    PhaseBatch b2(c, n * sizeof(PatStatic) + aux_work.bytes() + 8 * task_base.size() + 3 * 256 +
+                         sizeof(CopyDesc) * (verdict_words / kFillPiece + 8) + 512);
+    d_ps = b2.add(hs.data(), n);
+    const WorkTable waux = aux_work.to_device(b2);
+    const unsigned long long *d_task_base = b2.add(task_base);
+    if (verdict_words) b2.zero(verdict_region, verdict_words * sizeof(uint32_t));
+    b2.flush();
 
     // ---------------- auxiliary tables ----------------
     if (aux_work.base.back()) {
-        WorkTable waux = aux_work.to_device(c);
         const uint32_t gaux = grid_for(waux.total, c->num_cus);
         // no host round trip: an overflow of the pool is found with the
         // final results (the search kernels do nothing on an overflowed
@@ -628,7 +695,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         VerifyArgs va;
         va.ps = d_ps;
         va.st = d_st;
-        va.task_base = upload(c, task_base);
+        va.task_base = d_task_base;
         va.P = n;
         va.total = total_tasks;
         va.ctr = c->d_ctr;
@@ -668,7 +735,9 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                                total_tasks, feat);
         if (learned) {
             float *logits = c->work.alloc<float>(total_tasks);
-            float *sums = c->work.alloc<float>((size_t)n * kMaxV);
+            // the per-vertex sums and their probability-level split, zeroed together
+            float *sums = c->work.alloc<float>((size_t)n * kMaxV * (1 + kPrioQ));
+            float *qsums = sums + (size_t)n * kMaxV;
             uint32_t *keys = c->work.alloc<uint32_t>(total_tasks);
             // counting sort: pg blocks own contiguous task chunks (pg <= 128)
             const uint32_t pg = (uint32_t)std::min<unsigned long long>(128, (total_tasks + kBlock - 1) / kBlock);
@@ -678,10 +747,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             uint32_t *pu = c->work.alloc<uint32_t>(total_tasks);          // (pattern << 4) | vertex per task
             uint32_t *order_pu = c->work.alloc<uint32_t>(total_tasks);
             if (n >= (1u << 28)) throw ArgError{TFSM_ERR_LIMIT, "more than 2^28 patterns in one call"};
-            float *qsums = c->work.alloc<float>((size_t)n * kMaxV * kPrioQ);
             uint8_t *cutoff = c->work.alloc<uint8_t>((size_t)n * kMaxV);
-            HIPX(hipMemsetAsync(sums, 0, (size_t)n * kMaxV * sizeof(float), s));
-            HIPX(hipMemsetAsync(qsums, 0, (size_t)n * kMaxV * kPrioQ * sizeof(float), s));
+            HIPX(hipMemsetAsync(sums, 0, (size_t)n * kMaxV * (1 + kPrioQ) * sizeof(float), s));
             if (feat) {
                 // traced: the feature rows are kept for training
                 hipLaunchKernelGGL(k_mlp_forward, dim3(gt), dim3(kBlock), 0, s, feat, total_tasks, c->d_rank_w,
