@@ -219,6 +219,7 @@ struct tfsm_ctx {
     float *d_feat = nullptr;                 // features of the last traced call
     uint64_t feat_len = 0, feat_cap = 0;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev_sync = nullptr;           // end of the stream's work, polled by sync_spin
     tfsm_call_stats stats{};
     std::string err;
 };
@@ -435,6 +436,17 @@ struct EvalCall {
     tfsm_pattern_result *out;
 };
 
+// Waits for the stream by polling an event.  A blocking wait adds tens of
+// microseconds of wake-up latency, paid two or three times per eval call on
+// the critical path; polling costs one host core while the GPU works.
+void sync_spin(tfsm_ctx *c) {
+    HIPX(hipEventRecord(c->ev_sync, c->stream));
+    hipError_t e;
+    while ((e = hipEventQuery(c->ev_sync)) == hipErrorNotReady) {
+    }
+    HIPX(e);
+}
+
 int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     const uint32_t n = call.n;
     const tfsm_eval_options &opt = call.opt;
@@ -461,7 +473,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     }
     c->work.reset();
     hipStream_t s = c->stream;
-    HIPX(hipStreamSynchronize(s));      // nothing of an earlier call still reads the staging
+    sync_spin(c);                       // nothing of an earlier call still reads the staging
     c->stage.reset();
     HIPX(hipEventRecord(c->ev[0], s));
 
@@ -601,7 +613,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         HIPX(hipMemcpyAsync(hst, d_st, n * sizeof(PatState), hipMemcpyDeviceToHost, s));
         hf[0] = 0;
         if (passes) HIPX(hipMemcpyAsync(hf, c->d_pflags + passes - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HIPX(hipStreamSynchronize(s));
+        sync_spin(c);
         if (hf[0] == 0) break;
     }
     HIPX(hipEventRecord(c->ev[1], s));
@@ -851,7 +863,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     if (total_tasks) HIPX(hipMemcpyAsync(hctr, c->d_ctr, sizeof(SearchCounters), hipMemcpyDeviceToHost, s));
     uint32_t *hpool = hf + 4;                // pool used, pool overflow
     HIPX(hipMemcpyAsync(hpool, c->d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HIPX(hipStreamSynchronize(s));
+    sync_spin(c);
     if (hpool[1]) {
         // the tables did not fit: grow the pool to twice the measured need and redo the call
         if (attempt >= 4) throw ArgError{TFSM_ERR_LIMIT, "table pool overflow"};
@@ -958,7 +970,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     c->last_state.assign(hst, hst + n);
     c->last_pool_used = hpool[0];
     c->last_decided = decided;
-    HIPX(hipStreamSynchronize(s));
+    sync_spin(c);
     return TFSM_OK;
 }
 
@@ -1012,6 +1024,7 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         c->verify_blocks16 = std::max(1, c->verify_blocks16);
         HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto &e : c->ev) HIPX(hipEventCreate(&e));
+        HIPX(hipEventCreateWithFlags(&c->ev_sync, hipEventDisableTiming));
         c->n = g->num_vertices;
         c->L = g->num_vlabels;
         c->EL = g->num_elabels;
@@ -1122,6 +1135,7 @@ int tfsm_ctx_destroy(tfsm_ctx *c) {
     (void)hipFree(c->d_feat);
     (void)hipFree(c->d_trace);
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->ev_sync) (void)hipEventDestroy(c->ev_sync);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return TFSM_OK;

@@ -195,6 +195,17 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
         }
         std::vector<tfsm_pattern_result> res(n);
         std::memset(res.data(), 0, n * sizeof(tfsm_pattern_result));
+        // the next level's candidates are grown for every pattern of this
+        // level while the GPU evaluates it (growth is pure: a pattern's
+        // children depend on the pattern alone); only the frequent ones'
+        // children are kept, so the candidates are those of growing after
+        // the verdicts
+        std::vector<std::vector<Pattern>> kids(n);
+        std::thread grower([&]() { parallel_for(n, opt.threads, [&](size_t i) { kids[i] = extend(level[i].p, gc); }); });
+        struct Join {
+            std::thread &t;
+            ~Join() { if (t.joinable()) t.join(); }
+        } join_grower{grower};
         auto account = [&]() {
             tfsm_call_stats cs;
             tfsm_last_stats(ctx, &cs);
@@ -246,8 +257,7 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
             if (res[i].domain_handle) live_handles.push_back(res[i].domain_handle);
         }
         ls.frequent = freq_idx.size();
-        std::vector<std::vector<Pattern>> kids(freq_idx.size());
-        parallel_for(freq_idx.size(), opt.threads, [&](size_t k) { kids[k] = extend(level[freq_idx[k]].p, gc); });
+        grower.join();
         std::vector<Candidate> next;
         for (size_t k = 0; k < freq_idx.size(); ++k) {
             const size_t i = freq_idx[k];
@@ -256,7 +266,7 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
             f.support_lower = res[i].support_lower;
             f.support_upper = res[i].support_upper;
             found.push_back(std::move(f));
-            for (auto &c : kids[k]) next.push_back(Candidate{std::move(c), res[i].domain_handle});
+            for (auto &c : kids[i]) next.push_back(Candidate{std::move(c), res[i].domain_handle});
         }
         stats.frequent += freq_idx.size();
         ls.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tl).count();
