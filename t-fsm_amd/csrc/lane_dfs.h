@@ -120,12 +120,34 @@ struct Lane {
     bool adaptive_rt = false;      // the pattern's order (kOrderMixed)
     TFSM_LANE_FN bool adaptive() const { return ORD == kOrderAdaptive || (ORD == kOrderMixed && adaptive_rt); }
     uint32_t pf_pos = kUnknown, pf_val = 0;   // prefetched pool entry (the pool is read-only)
-    uint32_t ep[kCutMax / 4] = {};            // adaptive order: the pattern's packed edges
+    // Adaptive order: the pattern's constants, loaded once per task (the
+    // search kernel's occupancy is set by its LDS, so registers are spare):
+    // the packed edges, where each slot's row pairs start, and for patterns
+    // of up to 8 vertices the same-label masks (8 bits per vertex).
+    uint32_t ep[kCutMax / 4] = {};
+    uint32_t aor[2 * kCutMax] = {};
+    unsigned long long slab = 0;     // a value, not an array: a run-time index would put it in scratch
 
     template <class A>
     TFSM_LANE_FN void load_edges(const A &a) {
 #pragma unroll
         for (uint32_t i = 0; i < kCutMax / 4; ++i) ep[i] = a.edge_pack(i);
+#pragma unroll
+        for (uint32_t e = 0; e < kCutMax; ++e) {
+            // slots of absent edges are never selected; slot 0 always exists
+            const bool edge = ((ep[e >> 2] >> ((e & 3u) * 8u)) & 0xffu) != 0xffu;
+            aor[2 * e] = a.cut_ao(edge ? 2 * e : 0u);
+            aor[2 * e + 1] = a.cut_ao(edge ? 2 * e + 1 : 0u);
+        }
+        if (MV <= 8) {
+            slab = 0;
+            for (uint32_t x = 0; x < n && x < 8; ++x) slab |= (unsigned long long)(a.same_lab(x) & 0xffu) << (8u * x);
+        }
+    }
+    template <class A>
+    TFSM_LANE_FN uint32_t same_lab_of(const A &a, uint32_t x) const {
+        if (MV <= 8) return (uint32_t)(slab >> (8u * x)) & 0xffu;
+        return a.same_lab(x);
     }
 
     // Reference order: rows of depth dd, read once with their loads in
@@ -199,7 +221,7 @@ struct Lane {
         }
         uint32_t ao[kCutMax];
 #pragma unroll
-        for (uint32_t e = 0; e < kCutMax; ++e) ao[e] = a.cut_ao(sel[e]);
+        for (uint32_t e = 0; e < kCutMax; ++e) ao[e] = (sel[e] & 1u) ? aor[2 * e + 1] : (sel[e] ? aor[2 * e] : aor[0]);
         uint32_t lo[kCutMax], hi[kCutMax];
 #pragma unroll
         for (uint32_t e = 0; e < kCutMax; ++e) a.row_at(ao[e], rr[e], lo[e], hi[e]);
@@ -264,7 +286,7 @@ struct Lane {
             load_edges(a);
             vis = 1u << u;
             c.meta(0) = meta_pack(0, u, 0, 0);
-            if (a.same_lab(u)) c.vid(u) = a.vid(u, ci);
+            if (same_lab_of(a, u)) c.vid(u) = a.vid(u, ci);
             enter_adaptive(c, a, 1, kNone, kNone);
         } else {
             c.vid(0) = a.vid(u, ci);
@@ -290,7 +312,7 @@ struct Lane {
                 const uint32_t w = pord[j] & 15u;
                 c.idx(w) = prefix[j];
                 c.meta(j) = meta_pack(0, w, 0, 0);
-                if (a.same_lab(w)) c.vid(w) = a.vid(w, prefix[j]);
+                if (same_lab_of(a, w)) c.vid(w) = a.vid(w, prefix[j]);
                 vis |= 1u << w;
             }
             enter_adaptive(c, a, floor_, pord[floor_] & 15u, prim);
@@ -338,7 +360,7 @@ struct Lane {
             w = meta_vertex(m);
             nb = meta_secn(m) + 1;
             g0 = meta_secb(m);
-            sl = a.same_lab(w);
+            sl = same_lab_of(a, w);
         } else {
             w = a.order(u, d);
             nb = a.bn_n(u, d);
