@@ -158,6 +158,7 @@ struct VerifyArgs {
     SearchCounters *ctr;
     const uint32_t *pool;
     const uint32_t *aux_region;            // every slot's row offsets of the call
+    uint32_t max_edges;                    // the most edges of any searched pattern of the call
     const uint32_t *overflow;              // nonzero: the tables did not fit the pool (do nothing)
     uint32_t *trace;                       // nullable, indexed by reference task order
     const uint32_t *order;                 // nullable: search order -> reference task index

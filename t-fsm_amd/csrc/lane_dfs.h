@@ -110,7 +110,8 @@ TFSM_LANE_FN uint32_t lane_gallop(const A &a, uint32_t lo, uint32_t hi, uint32_t
 // the row offsets of slot s start; slot 2e is edge e's table from x, over
 // C(x), and 2e + 1 the one from y) and row_at(ao, r, lo, hi) (row r of the
 // table whose offsets start at ao), same_lab(x) (other vertices with x's
-// label).
+// label), edge_max() (at least the edge count of every pattern the
+// caller serves; uniform on the device).
 // C: idx(x), vid(j), pos(d), end(d), cur(g), cend(g), curv(g), meta(d), all
 // lvalues.
 template <int MV, int ORD = kOrderMixed>
@@ -132,8 +133,10 @@ struct Lane {
     TFSM_LANE_FN void load_edges(const A &a) {
 #pragma unroll
         for (uint32_t i = 0; i < kCutMax / 4; ++i) ep[i] = a.edge_pack(i);
+        const uint32_t me = a.edge_max();
 #pragma unroll
         for (uint32_t e = 0; e < kCutMax; ++e) {
+            if (e >= me) continue;
             // slots of absent edges are never selected; slot 0 always exists
             const bool edge = ((ep[e >> 2] >> ((e & 3u) * 8u)) & 0xffu) != 0xffu;
             aor[2 * e] = a.cut_ao(edge ? 2 * e : 0u);
@@ -202,10 +205,14 @@ struct Lane {
     // arrive in one round trip instead of one per edge.
     template <class C, class A>
     TFSM_LANE_FN void enter_adaptive(C &c, const A &a, uint32_t dd, uint32_t w, uint32_t prim) {
+        // me: the most edges of any pattern of the launch (uniform), so the
+        // wave skips the slots no pattern has
+        const uint32_t me = a.edge_max();
         uint32_t sel[kCutMax], rr[kCutMax];
         uint32_t cut = 0, dlo = 0, dhi = 0;     // cut edges; their unmatched ends, 4 bits each
 #pragma unroll
         for (uint32_t e = 0; e < kCutMax; ++e) {
+            if (e >= me) continue;
             const uint32_t xy = (ep[e >> 2] >> ((e & 3u) * 8u)) & 0xffu;
             const bool edge = xy != 0xffu;
             const uint32_t x = edge ? xy & 15u : 0u, y = edge ? xy >> 4 : 0u;
@@ -221,16 +228,23 @@ struct Lane {
         }
         uint32_t ao[kCutMax];
 #pragma unroll
-        for (uint32_t e = 0; e < kCutMax; ++e) ao[e] = (sel[e] & 1u) ? aor[2 * e + 1] : (sel[e] ? aor[2 * e] : aor[0]);
+        for (uint32_t e = 0; e < kCutMax; ++e) {
+            if (e >= me) continue;
+            ao[e] = (sel[e] & 1u) ? aor[2 * e + 1] : (sel[e] ? aor[2 * e] : aor[0]);
+        }
         uint32_t lo[kCutMax], hi[kCutMax];
 #pragma unroll
-        for (uint32_t e = 0; e < kCutMax; ++e) a.row_at(ao[e], rr[e], lo[e], hi[e]);
+        for (uint32_t e = 0; e < kCutMax; ++e) {
+            if (e >= me) continue;
+            a.row_at(ao[e], rr[e], lo[e], hi[e]);
+        }
         auto dst_of = [&](uint32_t e) { return e < 8 ? (dlo >> (4u * e)) & 15u : (dhi >> (4u * (e - 8))) & 15u; };
         auto len_of = [&](uint32_t e) { return ((cut >> e) & 1u) ? hi[e] - lo[e] : 0xffffffffu; };
         if (w == kNone) {
             uint32_t best = 0xffffffffu;
 #pragma unroll
             for (uint32_t e = 0; e < kCutMax; ++e) {
+                if (e >= me) continue;
                 const uint32_t l = len_of(e);
                 if (((cut >> e) & 1u) && l < best) {
                     best = l;
@@ -240,7 +254,8 @@ struct Lane {
         }
         uint32_t nb = 0, pick = 0, plen = 0xffffffffu;
 #pragma unroll
-        for (uint32_t e = 0; e < kCutMax; ++e)
+        for (uint32_t e = 0; e < kCutMax; ++e) {
+            if (e >= me) continue;
             if (((cut >> e) & 1u) && dst_of(e) == w) {
                 if (len_of(e) < plen) {
                     plen = len_of(e);
@@ -248,12 +263,14 @@ struct Lane {
                 }
                 ++nb;
             }
+        }
         if (prim == kNone) prim = pick;
         const uint32_t pm = dd ? c.meta(dd - 1) : 0u;
         const uint32_t secb = dd ? meta_secb(pm) + meta_secn(pm) : 0u;
         uint32_t k = 0, g = secb, plo = 0, phi = 0;
 #pragma unroll
-        for (uint32_t e = 0; e < kCutMax; ++e)
+        for (uint32_t e = 0; e < kCutMax; ++e) {
+            if (e >= me) continue;
             if (((cut >> e) & 1u) && dst_of(e) == w) {
                 if (k == prim) {
                     plo = lo[e];
@@ -268,6 +285,7 @@ struct Lane {
                 }
                 ++k;
             }
+        }
         c.pos(dd) = plo;                     // no rows: the depth is exhausted at once
         c.end(dd) = phi;
         c.meta(dd) = meta_pack(prim, w, secb, nb ? nb - 1 : 0);

@@ -670,11 +670,19 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     // work, the task bases) in one copy, the verdict bitmaps' fill in one
     // launch; the later kernels take the new descriptor copy
     PhaseBatch b2(c, n * sizeof(PatStatic) + aux_work.bytes() + 8 * task_base.size() + 3 * 256 +
-                         sizeof(CopyDesc) * (verdict_words / kFillPiece + 8) + 512);
+                         sizeof(CopyDesc) * (verdict_words / kFillPiece + 16) + 512);
     d_ps = b2.add(hs.data(), n);
     const WorkTable waux = aux_work.to_device(b2);
     const unsigned long long *d_task_base = b2.add(task_base);
     if (verdict_words) b2.zero(verdict_region, verdict_words * sizeof(uint32_t));
+    // the learned order's per-vertex sums (and their probability-level
+    // split) and the search counters start at zero
+    float *rank_sums = nullptr;
+    if (learned && task_base.back()) {
+        rank_sums = c->work.alloc<float>((size_t)n * kMaxV * (1 + kPrioQ));
+        b2.zero(rank_sums, (size_t)n * kMaxV * (1 + kPrioQ) * sizeof(float));
+    }
+    if (task_base.back()) b2.zero(c->d_ctr, sizeof(SearchCounters));
     b2.flush();
 
     // ---------------- auxiliary tables ----------------
@@ -747,8 +755,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                                total_tasks, feat);
         if (learned) {
             float *logits = c->work.alloc<float>(total_tasks);
-            // the per-vertex sums and their probability-level split, zeroed together
-            float *sums = c->work.alloc<float>((size_t)n * kMaxV * (1 + kPrioQ));
+            float *sums = rank_sums;
             float *qsums = sums + (size_t)n * kMaxV;
             uint32_t *keys = c->work.alloc<uint32_t>(total_tasks);
             // counting sort: pg blocks own contiguous task chunks (pg <= 128)
@@ -759,8 +766,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             uint32_t *pu = c->work.alloc<uint32_t>(total_tasks);          // (pattern << 4) | vertex per task
             uint32_t *order_pu = c->work.alloc<uint32_t>(total_tasks);
             if (n >= (1u << 28)) throw ArgError{TFSM_ERR_LIMIT, "more than 2^28 patterns in one call"};
-            uint8_t *cutoff = c->work.alloc<uint8_t>((size_t)n * kMaxV);
-            HIPX(hipMemsetAsync(sums, 0, (size_t)n * kMaxV * (1 + kPrioQ) * sizeof(float), s));
+            // with no margin the cutoff is computed inside k_priority
+            uint8_t *cutoff = kDeferMargin > 0.f ? c->work.alloc<uint8_t>((size_t)n * kMaxV) : nullptr;
             if (feat) {
                 // traced: the feature rows are kept for training
                 hipLaunchKernelGGL(k_mlp_forward, dim3(gt), dim3(kBlock), 0, s, feat, total_tasks, c->d_rank_w,
@@ -771,8 +778,9 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                 hipLaunchKernelGGL(k_rank_scores, dim3(gt), dim3(kBlock), 0, s, c->g, d_ps, va.task_base, n,
                                    total_tasks, c->d_rank_w, logits, sums, qsums);
             }
-            hipLaunchKernelGGL(k_defer_cutoff, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d_ps, n, sums, qsums,
-                               cutoff);
+            if (cutoff)
+                hipLaunchKernelGGL(k_defer_cutoff, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d_ps, n, sums,
+                                   qsums, cutoff);
             hipLaunchKernelGGL(k_priority, dim3(pg), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, chunk,
                                logits, sums, cutoff, keys, pu, hist);
             const uint32_t nhist = kPrioBins * pg, ntiles = (nhist + kScanTile - 1) / kScanTile;
@@ -792,14 +800,15 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             va.split = hist + (size_t)kDeferredKey * pg;
         }
         uint32_t epoch = ++c->epoch;
-        HIPX(hipMemsetAsync(c->d_ctr, 0, sizeof(SearchCounters), s));
         HIPX(hipEventRecord(c->ev[4], s));
         uint32_t max_nv = 0;
         bool all_adaptive = true;       // the instance without the reference order's code
+        va.max_edges = 0;
         for (uint32_t p = 0; p < n; ++p)
             if (decided[p] == TFSM_DECIDED_SEARCH) {
                 max_nv = std::max(max_nv, hs[p].nv);
                 all_adaptive = all_adaptive && hs[p].adaptive;
+                va.max_edges = std::max(va.max_edges, hs[p].nslots / 2);
             }
         // persistent grid: at most the resident blocks per CU (registers and
         // LDS: 52 KB per block for the 8-deep variant, 104 KB for the 16-deep one)

@@ -80,6 +80,7 @@ struct HostAcc {
         if (u >= n || d >= n) violation(what, u, d);
     }
     uint32_t nv() const { return n; }
+    uint32_t edge_max() const { return (uint32_t)edges.size(); }
     bool adaptive() const { return adaptive_; }
     uint32_t pool(uint32_t pos) const {
         if (pos >= pool_.size()) violation("pool", pos, pool_.size());
