@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "graph.h"
+#include "csr_cache.h"
 #include "miner.h"
 #include "pattern.h"
 #include "ranker.h"
@@ -26,6 +27,21 @@ tfsmh_graph *tfsmh_load(const char *path, int freq) {
     try {
         auto *h = new tfsmh_graph;
         h->g = tfsm::prune_graph(tfsm::read_lg(path), freq);
+        return h;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+// As tfsmh_load, through the binary cache beside the file (csr_cache.h);
+// *hit (when given) is 1 when the cache was used.
+tfsmh_graph *tfsmh_load_cached(const char *path, int freq, int *hit) {
+    try {
+        auto *h = new tfsmh_graph;
+        bool used = false;
+        h->g = tfsm::load_graph_cached(path, freq, &used);
+        if (hit) *hit = used ? 1 : 0;
         return h;
     } catch (const std::exception &e) {
         g_err = e.what();

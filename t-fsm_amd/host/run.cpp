@@ -3,11 +3,14 @@
 //   run -file G.lg -freq F [-thread T] [-maxNodes K]
 //       [-gpu plain|batched] [-support capped|exact] [-learned RANKER]
 //       [-tune TUNING.json] [-out grami_results.txt] [-stats stats.json]
+//       [-csrCache 1]
 //
 // Same flags and console summary as the reference (parallel/run.cpp:11-82);
 // flags are read by name in any order (the reference reads argv[2] as the
 // file).  -learned takes a ranker weight file written by the ranker trainer,
-// -tune a search-kernel shape written by the autotuner.
+// -tune a search-kernel shape written by the autotuner.  -csrCache 1 loads
+// the pruned graph from a binary cache beside the file (`<file>.f<freq>.csr`,
+// written on the first run and checked against the file's bytes).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -15,6 +18,7 @@
 #include <iostream>
 #include <string>
 
+#include "csr_cache.h"
 #include "graph.h"
 #include "miner.h"
 #include "ranker.h"
@@ -60,7 +64,9 @@ int main(int argc, char **argv) {
     auto t1 = clk::now();
     tfsm::DataGraph g;
     try {
-        g = tfsm::prune_graph(tfsm::read_lg(file), opt.freq);
+        const char *cache = flag(argc, argv, "-csrCache");
+        if (cache && std::strcmp(cache, "0") != 0) g = tfsm::load_graph_cached(file, opt.freq);
+        else g = tfsm::prune_graph(tfsm::read_lg(file), opt.freq);
     } catch (const std::exception &e) {
         std::cerr << e.what() << "\n";
         return 1;

@@ -162,6 +162,8 @@ def host_lib():
         vp = ctypes.c_void_p
         lib.tfsmh_load.argtypes = [ctypes.c_char_p, ctypes.c_int]
         lib.tfsmh_load.restype = vp
+        lib.tfsmh_load_cached.argtypes = [ctypes.c_char_p, ctypes.c_int, P(ctypes.c_int)]
+        lib.tfsmh_load_cached.restype = vp
         lib.tfsmh_free.argtypes = [vp]
         lib.tfsmh_info.argtypes = [vp, P(ctypes.c_uint64)]
         lib.tfsmh_csr.argtypes = [vp, P(ctypes.c_uint32), P(ctypes.c_uint32), P(ctypes.c_uint32),
@@ -192,9 +194,18 @@ def _check(ctx, rc):
 class Graph:
     """Pruned data graph loaded and relabelled by the product's host loader."""
 
-    def __init__(self, path, freq):
+    def __init__(self, path, freq, cache=False):
+        """cache: load through the binary cache beside the file
+        (`<path>.f<freq>.csr`, written when absent or stale); `cache_hit`
+        then says whether it was used."""
         self.lib = host_lib()
-        self.h = self.lib.tfsmh_load(path.encode(), int(freq))
+        self.cache_hit = False
+        if cache:
+            hit = ctypes.c_int(0)
+            self.h = self.lib.tfsmh_load_cached(path.encode(), int(freq), ctypes.byref(hit))
+            self.cache_hit = bool(hit.value)
+        else:
+            self.h = self.lib.tfsmh_load(path.encode(), int(freq))
         if not self.h:
             raise TfsmError(-1, self.lib.tfsmh_last_error().decode())
         info = (ctypes.c_uint64 * 5)()
