@@ -107,3 +107,27 @@ def test_rmat20_pattern_sets(name, workdir):
               flush=True)
     ctx.close()
     g.close()
+
+
+@pytest.mark.parametrize("learned", [False, True], ids=["batched", "learned"])
+def test_powerlaw_f6000_m6_pattern_sets(learned, workdir):
+    """The batched configuration: powerlaw100k at freq 6000, maxNodes 6 (631
+    candidate patterns over five levels, 334 frequent).  Batched runs, with
+    and without the learned order, find the reference's pattern set.  The
+    golden file holds the patterns of three or more vertices; the run also
+    reports the four frequent single edges, as the reference's total does."""
+    name = "powerlaw100k_f6000_m6"
+    m = golden_manifest()[name]
+    path = workload_path(m["workload"], m["seed"], workdir)
+    g = tfsm.Graph(path, m["freq"])
+    ctx = tfsm.Context(g)
+    if learned:
+        ctx.ranker_load(os.path.join(os.path.dirname(tfsm.__file__), "rankers", "powerlaw.f32"))
+    kp = os.path.join(str(workdir), f"{name}_{int(learned)}.keys")
+    n, stats = ctx.mine(m["freq"], m["max_nodes"], threads=4, batched=True, learned=learned, mode=tfsm.MODE_CAPPED,
+                        keys_path=kp)
+    assert n == m["reported_total"]
+    keys = set(read_keys(kp))
+    assert set(k for k in keys if k.count(";") > 1) == golden_keys(name)
+    assert len(keys) - len(golden_keys(name)) == 4
+    ctx.close()
