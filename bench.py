@@ -13,6 +13,10 @@ beside it.  The CPU baseline is the reference's own parallel build (compiled
 from its sources into oracle/_ref), timed once on this host's cores.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+
+`--workload powerlaw100k_f6000_m6` times the batched configuration instead
+(the same graph at freq 6000, maxNodes 6: 631 candidate patterns, 334
+frequent); its reference run takes minutes, so it has no CPU baseline here.
 """
 import argparse
 import json
@@ -25,10 +29,18 @@ import time
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "t-fsm_amd"))
 
-WORKLOAD = dict(name="powerlaw100k", seed=2, freq=9000, max_nodes=5, vertices=100000, edges=1100000)
+WORKLOADS = {
+    # golden: the reference's pattern set; min_vertices: the smallest patterns it lists
+    "powerlaw100k_f9000_m5": dict(name="powerlaw100k", seed=2, freq=9000, max_nodes=5, vertices=100000,
+                                  edges=1100000, label="powerlaw100k_chunglu_freq9000_maxnodes5", min_vertices=2,
+                                  tuning="powerlaw100k_f9000_m5.json"),
+    "powerlaw100k_f6000_m6": dict(name="powerlaw100k", seed=2, freq=6000, max_nodes=6, vertices=100000,
+                                  edges=1100000, label="powerlaw100k_chunglu_freq6000_maxnodes6", min_vertices=3,
+                                  tuning=None),
+}
+WORKLOAD = WORKLOADS["powerlaw100k_f9000_m5"]
 HBM_PEAK_GBPS = 8000.0
 RANKER = os.path.join(REPO, "t-fsm_amd", "rankers", "powerlaw.f32")
-TUNING = os.path.join(REPO, "t-fsm_amd", "tuning", "powerlaw100k_f9000_m5.json")
 
 
 def host_threads():
@@ -39,17 +51,19 @@ def host_threads():
     return max(1, min(16, n))
 
 
-def golden_keys():
-    path = os.path.join(REPO, "tests", "golden", "powerlaw100k_f9000_m5.keys")
+def golden_keys(workload):
+    path = os.path.join(REPO, "tests", "golden", workload + ".keys")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         return set(l.strip() for l in f if l.strip())
 
 
-def read_keys(path):
+def read_keys(path, min_vertices=2):
+    # a key lists its edges, each ended by ';': a pattern of v >= 3 vertices has >= 2 edges
     with open(path) as f:
-        return set(l.split("\t")[0] for l in f if l.strip())
+        keys = set(l.split("\t")[0] for l in f if l.strip())
+    return keys if min_vertices <= 2 else set(k for k in keys if k.count(";") >= min_vertices - 1)
 
 
 def cpu_baseline(lg, threads):
@@ -81,9 +95,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--paths", default="plain,batched,learned",
                     help="paths to time (comma list of plain, batched, learned; the value needs learned)")
+    ap.add_argument("--workload", default="powerlaw100k_f9000_m5", choices=sorted(WORKLOADS))
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()
+    global WORKLOAD
+    WORKLOAD = WORKLOADS[args.workload]
+    if args.workload != "powerlaw100k_f9000_m5":
+        args.no_cpu_baseline = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards across patterns "
               "and (u, v) tasks, see DESIGN.md, but only the one-GPU share is built and timed)", file=sys.stderr)
@@ -101,8 +120,9 @@ def main():
     ctx.ranker_load(RANKER)
     # the search-kernel shape chosen by t-fsm_amd/autotune.py for this workload
     # (library defaults when the file is absent); the same for every path
-    tuning = tfsm.load_tuning(TUNING) if os.path.exists(TUNING) else {}
-    gold = golden_keys()
+    tuning_file = os.path.join(REPO, "t-fsm_amd", "tuning", WORKLOAD["tuning"]) if WORKLOAD["tuning"] else None
+    tuning = tfsm.load_tuning(tuning_file) if tuning_file and os.path.exists(tuning_file) else {}
+    gold = golden_keys(args.workload)
     threads = host_threads()
 
     def run(batched, learned, steps, warmup, tag):
@@ -121,7 +141,7 @@ def main():
                 agg[k] += st[k]
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
-        keys = read_keys(keys_path)
+        keys = read_keys(keys_path, WORKLOAD["min_vertices"])
         parity = gold is None or keys == gold
         return elapsed, agg, parity, n
 
@@ -179,7 +199,7 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic (seeded Chung-Lu generator, t-fsm_amd/host/gen.h)",
-        "config": {"workload": "powerlaw100k_chunglu_freq9000_maxnodes5", "vertices": WORKLOAD["vertices"],
+        "config": {"workload": WORKLOAD["label"], "vertices": WORKLOAD["vertices"],
                    "edges": WORKLOAD["edges"], "freq": WORKLOAD["freq"], "max_nodes": WORKLOAD["max_nodes"],
                    "path": "batched+learned", "ranker": "t-fsm_amd/rankers/powerlaw.f32 (trained on seeds 11, 12)",
                    "tuning": tuning, "host_threads": threads},
