@@ -776,13 +776,13 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                                    logits, sums, qsums);
             } else {
                 hipLaunchKernelGGL(k_rank_scores, dim3(gt), dim3(kBlock), 0, s, c->g, d_ps, va.task_base, n,
-                                   total_tasks, c->d_rank_w, logits, sums, qsums);
+                                   total_tasks, c->d_rank_w, logits, sums, qsums, pu);
             }
             if (cutoff)
                 hipLaunchKernelGGL(k_defer_cutoff, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d_ps, n, sums,
                                    qsums, cutoff);
             hipLaunchKernelGGL(k_priority, dim3(pg), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, chunk,
-                               logits, sums, cutoff, keys, pu, hist);
+                               logits, sums, cutoff, keys, pu, hist, feat == nullptr);
             const uint32_t nhist = kPrioBins * pg, ntiles = (nhist + kScanTile - 1) / kScanTile;
             static_assert((size_t)kPrioBins * 128 <= (size_t)kScanTile * kBlock, "scan of the sums is one block");
             uint32_t *tiles = c->work.alloc<uint32_t>(ntiles);
