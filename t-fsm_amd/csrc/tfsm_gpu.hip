@@ -590,6 +590,10 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     b1.zero(d_st, n * sizeof(PatState));
     b1.zero(c->d_flags, 2 * sizeof(uint32_t));        // pool used, pool overflow
     if (wsrc.total) b1.zero(c->d_pflags, kMaxPrunePasses * sizeof(uint32_t));
+    // per pass, the pattern vertices that lost candidates (three arrays in rotation)
+    const uint32_t nchg = n * kMaxV;
+    uint32_t *d_chg = c->work.alloc<uint32_t>(3 * (size_t)nchg);
+    b1.zero(d_chg, 3 * (size_t)nchg * sizeof(uint32_t));
     b1.flush();
     const uint32_t gsrc = grid_for(wsrc.total, c->num_cus);
     if (wsrc.total) hipLaunchKernelGGL(k_ldf, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc);
@@ -605,7 +609,9 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         for (uint32_t i = 0; wsrc.total && i < kPruneBatch; ++i, ++passes) {
             if (passes >= kMaxPrunePasses) throw ArgError{TFSM_ERR_LIMIT, "filter did not converge"};
             hipLaunchKernelGGL(k_prune, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc, c->d_pflags + passes,
-                               passes ? c->d_pflags + passes - 1 : nullptr);
+                               passes ? c->d_pflags + passes - 1 : nullptr,
+                               passes ? d_chg + (size_t)((passes + 2) % 3) * nchg : nullptr,
+                               d_chg + (size_t)(passes % 3) * nchg, d_chg + (size_t)((passes + 1) % 3) * nchg, nchg);
         }
         hipLaunchKernelGGL(k_compact, dim3((uint32_t)pair_p.size()), dim3(kBlock), 0, s, c->g, d_ps, d_st, d_pair_p,
                            d_pair_u);
