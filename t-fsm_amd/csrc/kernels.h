@@ -57,8 +57,10 @@ struct PatStatic {
     const uint32_t *src_invalid[kMaxV];
     uint32_t src_n[kMaxV];
     // domains
+    // per 32 data vertices, a pair of words: the membership bits of C(u),
+    // then the rank (index in C(u)) of the group's first member, so a table
+    // build reads both with one 8-byte load
     uint32_t *bits[kMaxV];
-    uint32_t *rank[kMaxV];
     uint32_t *dom[kMaxV];
     uint32_t dom_size[kMaxV];           // filled by the host once known
     uint32_t *valid_bits[kMaxV];

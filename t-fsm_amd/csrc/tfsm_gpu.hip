@@ -564,8 +564,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         size_t bo = 0, dofs = 0;
         for (uint32_t p = 0; p < n; ++p)
             for (uint32_t u = 0; u < hs[p].nv; ++u) {
-                hs[p].bits[u] = bits_region + bo; bo += c->W;
-                hs[p].rank[u] = bits_region + bo; bo += c->W;
+                hs[p].bits[u] = bits_region + bo; bo += 2 * (size_t)c->W;   // (bits, rank) pairs
                 hs[p].dom[u] = dom_region + dofs; dofs += std::max<uint32_t>(hs[p].src_n[u], 1);
             }
     }
