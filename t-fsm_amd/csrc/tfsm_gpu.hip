@@ -682,10 +682,11 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     if (verdict_words) b2.zero(verdict_region, verdict_words * sizeof(uint32_t));
     // the learned order's per-vertex sums (and their probability-level
     // split) and the search counters start at zero
+    // sums, then their probability-level split, then the sample pass's sums
     float *rank_sums = nullptr;
     if (learned && task_base.back()) {
-        rank_sums = c->work.alloc<float>((size_t)n * kMaxV * (1 + kPrioQ));
-        b2.zero(rank_sums, (size_t)n * kMaxV * (1 + kPrioQ) * sizeof(float));
+        rank_sums = c->work.alloc<float>((size_t)n * kMaxV * (2 + kPrioQ));
+        b2.zero(rank_sums, (size_t)n * kMaxV * (2 + kPrioQ) * sizeof(float));
     }
     if (task_base.back()) b2.zero(c->d_ctr, sizeof(SearchCounters));
     b2.flush();
@@ -762,6 +763,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             float *logits = c->work.alloc<float>(total_tasks);
             float *sums = rank_sums;
             float *qsums = sums + (size_t)n * kMaxV;
+            float *ssums = qsums + (size_t)n * kMaxV * kPrioQ;
             uint32_t *keys = c->work.alloc<uint32_t>(total_tasks);
             // counting sort: pg blocks own contiguous task chunks (pg <= 128)
             const uint32_t pg = (uint32_t)std::min<unsigned long long>(128, (total_tasks + kBlock - 1) / kBlock);
@@ -780,14 +782,20 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                 hipLaunchKernelGGL(k_group_sums, dim3(gt), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks,
                                    logits, sums, qsums);
             } else {
+                // every kSample-th task first, then the tasks of the patterns it
+                // does not expect frequent
+                const uint32_t gs = grid_for((total_tasks + kSample - 1) / kSample, c->num_cus);
+                hipLaunchKernelGGL(k_rank_scores, dim3(gs), dim3(kBlock), 0, s, c->g, d_ps, va.task_base, n,
+                                   total_tasks, c->d_rank_w, logits, ssums, qsums, pu, kSample, (const float *)nullptr);
                 hipLaunchKernelGGL(k_rank_scores, dim3(gt), dim3(kBlock), 0, s, c->g, d_ps, va.task_base, n,
-                                   total_tasks, c->d_rank_w, logits, sums, qsums, pu);
+                                   total_tasks, c->d_rank_w, logits, sums, qsums, pu, 1u, (const float *)ssums);
             }
             if (cutoff)
                 hipLaunchKernelGGL(k_defer_cutoff, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d_ps, n, sums,
                                    qsums, cutoff);
             hipLaunchKernelGGL(k_priority, dim3(pg), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, chunk,
-                               logits, sums, cutoff, keys, pu, hist, feat == nullptr);
+                               logits, sums, cutoff, keys, pu, hist, feat == nullptr,
+                               feat == nullptr ? (const float *)ssums : (const float *)nullptr);
             const uint32_t nhist = kPrioBins * pg, ntiles = (nhist + kScanTile - 1) / kScanTile;
             static_assert((size_t)kPrioBins * 128 <= (size_t)kScanTile * kBlock, "scan of the sums is one block");
             uint32_t *tiles = c->work.alloc<uint32_t>(ntiles);
