@@ -174,6 +174,7 @@ struct PinnedStage {
 
 constexpr uint32_t kMaxPrunePasses = 4096;   // filter passes of one call
 constexpr uint32_t kPruneBatch = 6;          // passes launched between convergence checks
+constexpr uint32_t kPrioBlocks = 512;        // most blocks of the priority sort (one histogram column each)
 
 }  // namespace
 
@@ -765,8 +766,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             float *qsums = sums + (size_t)n * kMaxV;
             float *ssums = qsums + (size_t)n * kMaxV * kPrioQ;
             uint32_t *keys = c->work.alloc<uint32_t>(total_tasks);
-            // counting sort: pg blocks own contiguous task chunks (pg <= 128)
-            const uint32_t pg = (uint32_t)std::min<unsigned long long>(128, (total_tasks + kBlock - 1) / kBlock);
+            // counting sort: pg blocks own contiguous task chunks (pg <= kPrioBlocks)
+            const uint32_t pg = (uint32_t)std::min<unsigned long long>(kPrioBlocks, (total_tasks + kBlock - 1) / kBlock);
             const uint32_t chunk = (uint32_t)((total_tasks + pg - 1) / pg);
             uint32_t *hist = c->work.alloc<uint32_t>((size_t)kPrioBins * pg);
             uint32_t *order = c->work.alloc<uint32_t>(total_tasks);
@@ -797,7 +798,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                                logits, sums, cutoff, keys, pu, hist, feat == nullptr,
                                feat == nullptr ? (const float *)ssums : (const float *)nullptr);
             const uint32_t nhist = kPrioBins * pg, ntiles = (nhist + kScanTile - 1) / kScanTile;
-            static_assert((size_t)kPrioBins * 128 <= (size_t)kScanTile * kBlock, "scan of the sums is one block");
+            static_assert((size_t)kPrioBins * kPrioBlocks <= (size_t)kScanTile * 4 * kBlock,
+                          "the tile sums fit the one-block scan (4 per thread)");
             uint32_t *tiles = c->work.alloc<uint32_t>(ntiles);
             hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, hist, nhist, tiles);
             hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, tiles, ntiles);
