@@ -625,6 +625,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     HIPX(hipEventRecord(c->ev[1], s));
 
     // ---------------- decisions and plans ----------------
+    const bool wave_phase = (opt.long_steps == 0 ? kDefaultLongSteps : opt.long_steps) != 0xffffffffu;
     std::vector<int> decided(n);
     size_t aux_words = 0, verdict_words = 0;
     for (uint32_t p = 0; p < n; ++p) {
@@ -638,7 +639,10 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         else if (mn < call.freq) decided[p] = TFSM_DECIDED_FILTER;            // isCandidateSetValid(freq)
         else decided[p] = TFSM_DECIDED_SEARCH;
         if (decided[p] != TFSM_DECIDED_SEARCH) continue;
-        make_plans(P, slot_of.data() + (size_t)p * kMaxV * kMaxV);
+        // the reference order's plans: read by its search and by the
+        // wave-cooperative phase, and kept for the inspection of a
+        // one-pattern call (tfsm_last_order); the adaptive order needs none
+        if (!P.adaptive || n == 1 || wave_phase) make_plans(P, slot_of.data() + (size_t)p * kMaxV * kMaxV);
         P.task_prefix[0] = 0;
         for (uint32_t u = 0; u < P.nv; ++u) {
             P.task_prefix[u + 1] = P.task_prefix[u] + P.dom_size[u];
