@@ -105,7 +105,7 @@ def main():
         args.no_cpu_baseline = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards across patterns "
-              "and (u, v) tasks, see DESIGN.md, but only the one-GPU share is built and timed)", file=sys.stderr)
+              "of a level and across (u, v) tasks, but only the one-GPU share is built and timed)", file=sys.stderr)
         sys.exit(2)
 
     import torch
