@@ -998,7 +998,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     c->last_state.assign(hst, hst + n);
     c->last_pool_used = hpool[0];
     c->last_decided = decided;
-    sync_spin(c);
+    // the retained-domain copies stay queued: the next call waits for the stream before it
+    // reuses the staging, and every function that reads device memory from the host syncs first
     return TFSM_OK;
 }
 
@@ -1189,6 +1190,8 @@ int tfsm_domain_read(tfsm_ctx *ctx, uint64_t handle, uint32_t u, uint32_t *out, 
     const Retained &R = it->second;
     if (u >= R.nv) return fail(ctx, TFSM_ERR_ARGUMENT, "pattern vertex out of range");
     return guarded(ctx, [&]() -> int {
+               // an eval call returns with its retained-domain copies still queued
+               HIPX(hipStreamSynchronize(ctx->stream));
         const uint32_t n = R.size[u];
         std::vector<uint32_t> dom(n), inv((n + 31) / 32);
         if (n) {
@@ -1220,6 +1223,8 @@ int64_t tfsm_last_domain(tfsm_ctx *ctx, uint32_t u, uint32_t *out, uint32_t cap)
     const PatStatic &P = ctx->last_static[0];
     const uint32_t n = P.dom_size[u];
     int rc = guarded(ctx, [&]() -> int {
+                 // an eval call returns with its retained-domain copies still queued
+                 HIPX(hipStreamSynchronize(ctx->stream));
         if (out && n) HIPX(hipMemcpy(out, P.dom[u], std::min(n, cap) * sizeof(uint32_t), hipMemcpyDeviceToHost));
         return TFSM_OK;
     });
@@ -1232,6 +1237,8 @@ int64_t tfsm_last_states(tfsm_ctx *ctx, uint32_t u, uint8_t *out, uint32_t cap) 
     const int decided = ctx->last_decided[0];
     const uint32_t n = P.dom_size[u];
     int rc = guarded(ctx, [&]() -> int {
+                 // an eval call returns with its retained-domain copies still queued
+                 HIPX(hipStreamSynchronize(ctx->stream));
         std::vector<uint32_t> vb((n + 31) / 32, 0), ib((n + 31) / 32, 0);
         if (decided == TFSM_DECIDED_SEARCH && n) {
             HIPX(hipMemcpy(vb.data(), P.valid_bits[u], vb.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -1258,6 +1265,8 @@ int64_t tfsm_last_aux(tfsm_ctx *ctx, uint32_t a, uint32_t b, uint32_t *off, uint
     const uint32_t rows = P.dom_size[a];
     const uint32_t total = ctx->last_state[0].aux_total[slot];
     int rc = guarded(ctx, [&]() -> int {
+                 // an eval call returns with its retained-domain copies still queued
+                 HIPX(hipStreamSynchronize(ctx->stream));
         // the device keeps each row as a (start, end) pair of pool positions;
         // callers get the table as a plain CSR
         std::vector<uint32_t> se(2 * (size_t)rows), pool(idx ? ctx->last_pool_used : 0u);
@@ -1292,6 +1301,8 @@ int64_t tfsm_read_trace(tfsm_ctx *ctx, uint32_t *out, uint64_t cap) {
     if (!ctx) return TFSM_ERR_ARGUMENT;
     const uint64_t n = ctx->trace_len;
     int rc = guarded(ctx, [&]() -> int {
+                 // an eval call returns with its retained-domain copies still queued
+                 HIPX(hipStreamSynchronize(ctx->stream));
         if (out && n) HIPX(hipMemcpy(out, ctx->d_trace, std::min(n, cap) * sizeof(uint32_t), hipMemcpyDeviceToHost));
         return TFSM_OK;
     });
@@ -1463,6 +1474,8 @@ int64_t tfsm_read_features(tfsm_ctx *ctx, float *out, uint64_t cap_rows) {
     if (!ctx) return TFSM_ERR_ARGUMENT;
     const uint64_t n = ctx->feat_len;
     int rc = guarded(ctx, [&]() -> int {
+                 // an eval call returns with its retained-domain copies still queued
+                 HIPX(hipStreamSynchronize(ctx->stream));
         if (out && n)
             HIPX(hipMemcpy(out, ctx->d_feat, std::min(n, cap_rows) * kFeat * sizeof(float), hipMemcpyDeviceToHost));
         return TFSM_OK;
