@@ -193,14 +193,13 @@ struct tfsm_ctx {
     Arena work;
     uint32_t *d_pool = nullptr;
     uint32_t pool_cap = 0;
-    uint32_t *d_flags = nullptr;             // [0] pool_used, [1] overflow
+    uint32_t *d_flags = nullptr;             // [0] pool_used, [1] overflow (after the two d_ctr entries)
     uint32_t *d_pflags = nullptr;            // per filter pass: 1 when it removed a candidate
     PinnedStage stage;                       // upload staging of the current call
-    SearchCounters *d_ctr = nullptr;
+    SearchCounters *d_ctr = nullptr;         // first and deferred search launch
     SplitRecord *d_records = nullptr;        // kSplitCap records, reused across launches by epoch
     uint32_t *d_piece_cnt = nullptr, *d_piece_inc = nullptr;
     uint32_t epoch = 0;
-    Pinned<uint32_t> h_flags;
     Pinned<PatState> h_state;
     Pinned<SearchCounters> h_ctr;
     // retained domains
@@ -569,7 +568,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                 hs[p].dom[u] = dom_region + dofs; dofs += std::max<uint32_t>(hs[p].src_n[u], 1);
             }
     }
-    PatState *d_st = c->work.alloc<PatState>(n);
+    // the pattern states, then one word: the filter's last change flag
+    PatState *d_st = c->work.alloc<PatState>(n + 1);
 
     // ---------------- filter: LDF, DP-iso to the fixpoint, compaction ----------------
     HostWork src_work;
@@ -582,12 +582,13 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         }
     // this phase's uploads in one copy, its fills in one launch
     PhaseBatch b1(c, n * sizeof(PatStatic) + src_work.bytes() + 2 * 256 + 8 * pair_p.size() +
-                         sizeof(CopyDesc) * (bits_words / kFillPiece + n * sizeof(PatState) / 4 / kFillPiece + 8) + 512);
+                         sizeof(CopyDesc) * (bits_words / kFillPiece + (n + 1) * sizeof(PatState) / 4 / kFillPiece + 8) +
+                         512);
     PatStatic *d_ps = b1.add(hs.data(), n);
     WorkTable wsrc = src_work.to_device(b1);
     const uint32_t *d_pair_p = b1.add(pair_p), *d_pair_u = b1.add(pair_u);
     b1.zero(bits_region, bits_words * sizeof(uint32_t));
-    b1.zero(d_st, n * sizeof(PatState));
+    b1.zero(d_st, (n + 1) * sizeof(PatState));
     b1.zero(c->d_flags, 2 * sizeof(uint32_t));        // pool used, pool overflow
     if (wsrc.total) b1.zero(c->d_pflags, kMaxPrunePasses * sizeof(uint32_t));
     // per pass, the pattern vertices that lost candidates (three arrays in rotation)
@@ -598,12 +599,13 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     const uint32_t gsrc = grid_for(wsrc.total, c->num_cus);
     if (wsrc.total) hipLaunchKernelGGL(k_ldf, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc);
     HIPX(hipGetLastError());
-    uint32_t *hf = c->h_flags.ensure(8);
     // DP-iso passes in batches without host round trips: a pass returns at
     // once when the one before it removed nothing (the fixpoint is reached);
     // the last flag of a batch, read with the compacted domains, says whether
     // another batch is needed (compaction is then simply redone)
-    PatState *hst = c->h_state.ensure(n);
+    PatState *hst = c->h_state.ensure(n + 1);
+    uint32_t *d_more = reinterpret_cast<uint32_t *>(d_st + n);
+    const uint32_t *h_more = reinterpret_cast<const uint32_t *>(hst + n);
     uint32_t passes = 0;
     for (;;) {
         for (uint32_t i = 0; wsrc.total && i < kPruneBatch; ++i, ++passes) {
@@ -614,13 +616,11 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                                d_chg + (size_t)(passes % 3) * nchg, d_chg + (size_t)((passes + 1) % 3) * nchg, nchg);
         }
         hipLaunchKernelGGL(k_compact, dim3((uint32_t)pair_p.size()), dim3(kBlock), 0, s, c->g, d_ps, d_st, d_pair_p,
-                           d_pair_u);
+                           d_pair_u, passes ? c->d_pflags + passes - 1 : (const uint32_t *)nullptr, d_more);
         HIPX(hipGetLastError());
-        HIPX(hipMemcpyAsync(hst, d_st, n * sizeof(PatState), hipMemcpyDeviceToHost, s));
-        hf[0] = 0;
-        if (passes) HIPX(hipMemcpyAsync(hf, c->d_pflags + passes - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPX(hipMemcpyAsync(hst, d_st, (n + 1) * sizeof(PatState), hipMemcpyDeviceToHost, s));
         sync_spin(c);
-        if (hf[0] == 0) break;
+        if (h_more[0] == 0) break;
     }
     HIPX(hipEventRecord(c->ev[1], s));
 
@@ -693,7 +693,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         rank_sums = c->work.alloc<float>((size_t)n * kMaxV * (2 + kPrioQ));
         b2.zero(rank_sums, (size_t)n * kMaxV * (2 + kPrioQ) * sizeof(float));
     }
-    if (task_base.back()) b2.zero(c->d_ctr, sizeof(SearchCounters));
+    if (task_base.back()) b2.zero(c->d_ctr, 2 * sizeof(SearchCounters));    // one per search launch
     b2.flush();
 
     // ---------------- auxiliary tables ----------------
@@ -867,10 +867,9 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         va.part = two_parts ? 1u : 0u;
         launch_verify();
         if (two_parts) {
-            // the first launch's counters (splits, watchdog) before they are cleared
-            HIPX(hipMemcpyAsync(c->h_ctr.ensure(2) + 1, c->d_ctr, sizeof(SearchCounters), hipMemcpyDeviceToHost, s));
+            // the deferred launch keeps its own counters, zeroed with the first's
             epoch = ++c->epoch;
-            HIPX(hipMemsetAsync(c->d_ctr, 0, sizeof(SearchCounters), s));
+            va.ctr = c->d_ctr + 1;
             va.part = 2;
             launch_verify();
         }
@@ -887,10 +886,10 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     }
     HIPX(hipEventRecord(c->ev[3], s));
     HIPX(hipMemcpyAsync(hst, d_st, n * sizeof(PatState), hipMemcpyDeviceToHost, s));
-    SearchCounters *hctr = c->h_ctr.ensure(2);
-    if (total_tasks) HIPX(hipMemcpyAsync(hctr, c->d_ctr, sizeof(SearchCounters), hipMemcpyDeviceToHost, s));
-    uint32_t *hpool = hf + 4;                // pool used, pool overflow
-    HIPX(hipMemcpyAsync(hpool, c->d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    // the two launches' counters and the pool words follow each other: one copy
+    SearchCounters *hctr = c->h_ctr.ensure(3);
+    HIPX(hipMemcpyAsync(hctr, c->d_ctr, 2 * sizeof(SearchCounters) + 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    const uint32_t *hpool = reinterpret_cast<const uint32_t *>(hctr + 2);   // pool used, pool overflow
     sync_spin(c);
     if (hpool[1]) {
         // the tables did not fit: grow the pool to twice the measured need and redo the call
@@ -1113,9 +1112,10 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         c->g.nbr = c->d_nbr;
         c->pool_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * g->num_adj, 1u << 22), 0xfffffff0ull);
         HIPX(hipMalloc(&c->d_pool, (size_t)c->pool_cap * sizeof(uint32_t)));
-        HIPX(hipMalloc(&c->d_flags, 16 * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_pflags, kMaxPrunePasses * sizeof(uint32_t)));
-        HIPX(hipMalloc(&c->d_ctr, sizeof(SearchCounters)));
+        // the counters of the two search launches, then the pool words
+        HIPX(hipMalloc(&c->d_ctr, 3 * sizeof(SearchCounters)));
+        c->d_flags = reinterpret_cast<uint32_t *>(c->d_ctr + 2);
         HIPX(hipMalloc(&c->d_records, (size_t)kSplitCap * sizeof(SplitRecord)));
         HIPX(hipMalloc(&c->d_piece_cnt, (size_t)kSplitCap * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_piece_inc, (size_t)kSplitCap * sizeof(uint32_t)));
@@ -1154,7 +1154,6 @@ int tfsm_ctx_destroy(tfsm_ctx *c) {
     (void)hipFree(c->d_nbr);
     (void)hipFree(c->d_bucket);
     (void)hipFree(c->d_pool);
-    (void)hipFree(c->d_flags);
     (void)hipFree(c->d_ctr);
     (void)hipFree(c->d_records);
     (void)hipFree(c->d_piece_cnt);

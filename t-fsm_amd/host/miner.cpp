@@ -95,14 +95,47 @@ public:
         cur_.reset();
     }
 
-private:
     struct Job {
         const std::function<void(size_t)> *fn = nullptr;
+        std::function<void(size_t)> own;     // a background job's function
         size_t n = 0;
         std::atomic<size_t> next{0}, left{0};
         std::mutex m;
         std::condition_variable cv;
     };
+    // A job on the workers alone: start() returns at once, so the caller can
+    // do other work meanwhile, and finish() waits for its items.  The pool
+    // runs no other job in between (the same thread must call both).
+    std::shared_ptr<Job> start(size_t n, int threads, std::function<void(size_t)> fn) {
+        run_mu_.lock();
+        auto job = std::make_shared<Job>();
+        job->own = std::move(fn);
+        job->fn = &job->own;
+        job->n = n;
+        job->left.store(n);
+        if (n == 0) return job;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            while ((int)workers_.size() < std::max(1, threads - 1)) workers_.emplace_back([this]() { loop(); });
+            cur_ = job;
+            ++gen_;
+        }
+        cv_.notify_all();
+        return job;
+    }
+    void finish(const std::shared_ptr<Job> &job) {
+        {
+            std::unique_lock<std::mutex> lk(job->m);
+            job->cv.wait(lk, [&]() { return job->left.load() == 0; });
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            cur_.reset();
+        }
+        run_mu_.unlock();
+    }
+
+private:
     static void drain(Job &job) {
         size_t i;
         while ((i = job.next.fetch_add(1)) < job.n) {
@@ -144,6 +177,22 @@ void parallel_for(size_t n, int threads, F &&f) {
     const std::function<void(size_t)> fn = [&](size_t i) { f(i); };
     WorkerPool::get().run(n, std::min<int>(threads, (int)n), fn);
 }
+
+// fn over [0, n) on the pool's workers while the caller goes on; wait() (or
+// the destructor) returns once every item ran.
+class Background {
+public:
+    Background(size_t n, int threads, std::function<void(size_t)> fn)
+        : job_(WorkerPool::get().start(n, std::min<int>(std::max(threads, 2), (int)n + 1), std::move(fn))) {}
+    ~Background() { wait(); }
+    void wait() {
+        if (job_) WorkerPool::get().finish(job_);
+        job_.reset();
+    }
+
+private:
+    std::shared_ptr<WorkerPool::Job> job_;
+};
 
 }  // namespace
 
@@ -201,11 +250,7 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
         // children are kept, so the candidates are those of growing after
         // the verdicts
         std::vector<std::vector<Pattern>> kids(n);
-        std::thread grower([&]() { parallel_for(n, opt.threads, [&](size_t i) { kids[i] = extend(level[i].p, gc); }); });
-        struct Join {
-            std::thread &t;
-            ~Join() { if (t.joinable()) t.join(); }
-        } join_grower{grower};
+        Background grower(n, opt.threads, [&](size_t i) { kids[i] = extend(level[i].p, gc); });
         auto account = [&]() {
             tfsm_call_stats cs;
             tfsm_last_stats(ctx, &cs);
@@ -257,7 +302,7 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
             if (res[i].domain_handle) live_handles.push_back(res[i].domain_handle);
         }
         ls.frequent = freq_idx.size();
-        grower.join();
+        grower.wait();
         std::vector<Candidate> next;
         for (size_t k = 0; k < freq_idx.size(); ++k) {
             const size_t i = freq_idx[k];
