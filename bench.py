@@ -36,7 +36,7 @@ WORKLOADS = {
                                   tuning="powerlaw100k_f9000_m5.json"),
     "powerlaw100k_f6000_m6": dict(name="powerlaw100k", seed=2, freq=6000, max_nodes=6, vertices=100000,
                                   edges=1100000, label="powerlaw100k_chunglu_freq6000_maxnodes6", min_vertices=3,
-                                  tuning=None),
+                                  tuning="powerlaw100k_f6000_m6.json"),
 }
 WORKLOAD = WORKLOADS["powerlaw100k_f9000_m5"]
 HBM_PEAK_GBPS = 8000.0
@@ -212,6 +212,13 @@ def main():
         "plain": side(plain_agg, plain_s, plain_value),
         "batched": side(batch_agg, batch_s, batch_value),
         "search_ms_per_step": round(agg["search_ms"] / K, 3),
+        # the auxiliary-table phase of the same path (host plans, their upload, k_aux_build) between two
+        # stream events, and the table build's algorithmic bytes (neighbour ids scanned, segment bounds
+        # read, row pairs and entries written)
+        "table_phase": {"ms_per_step": round(agg["table_ms"] / K, 3),
+                        "bytes_per_step": int(agg["bytes_table"] / K),
+                        "GB_per_s": round(agg["bytes_table"] / (agg["table_ms"] * 1e-3) / 1e9, 2)
+                        if agg["table_ms"] > 0 else None},
         "speedup_vs_plain": round(value / plain_value, 3) if plain_value else None,
         "speedup_batched_vs_plain": round(batch_value / plain_value, 3) if plain_value and batch_value else None,
         "roofline": {"bound": "hbm", "kernel": "k_verify (existence search)", "achieved": round(achieved, 3),
