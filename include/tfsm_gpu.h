@@ -149,7 +149,10 @@ int tfsm_ctx_set_pool_capacity(tfsm_ctx *ctx, uint64_t entries);
 int tfsm_eval_pattern(tfsm_ctx *ctx, const tfsm_pattern *p, uint64_t parent_handle, uint32_t freq,
                       const tfsm_eval_options *opt, tfsm_pattern_result *out);
 
-/* Every pattern of one extension level in one fused launch sequence. */
+/* Every pattern of one extension level in one fused launch sequence.
+ * Both eval calls return with the results on the host and the copies of the
+ * retained child domains still queued on the context's stream; every later
+ * call that reads device memory from the host waits for the stream first. */
 int tfsm_eval_level(tfsm_ctx *ctx, const tfsm_pattern *ps, const uint64_t *parent_handles, uint32_t n,
                     uint32_t freq, const tfsm_eval_options *opt, tfsm_pattern_result *outs);
 
