@@ -830,7 +830,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                 va.max_edges = std::max(va.max_edges, hs[p].nslots / 2);
             }
         // persistent grid: at most the resident blocks per CU (registers and
-        // LDS: 52 KB per block for the 8-deep variant, 104 KB for the 16-deep one)
+        // LDS: verify_lds_bytes(8) = 60 KB per block for the 8-deep variant,
+        // verify_lds_bytes(16) = 120 KB for the 16-deep one, of 160 KB per CU)
         const uint32_t resident =
             (uint32_t)(max_nv > 8 ? c->verify_blocks16 : all_adaptive ? c->verify_blocks8a : c->verify_blocks8);
         const uint32_t per_cu = opt.grid_waves_per_cu
@@ -1342,7 +1343,9 @@ int tfsm_ranker_set(tfsm_ctx *ctx, const tfsm_ranker_weights *w) {
             if (!std::isfinite(f[i])) throw ArgError{TFSM_ERR_ARGUMENT, "ranker weight is not finite"};
             h[i] = host_bf16(f[i]);
         }
-        HIPX(hipMemcpy(ctx->d_rank_w, h.data(), kParams * sizeof(uint16_t), hipMemcpyHostToDevice));
+        // queued work of an earlier call may still read the installed weights
+        HIPX(hipMemcpyAsync(ctx->d_rank_w, h.data(), kParams * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream));
+        HIPX(hipStreamSynchronize(ctx->stream));
         ctx->ranker_installed = true;
         return TFSM_OK;
     });
@@ -1353,7 +1356,8 @@ int tfsm_ranker_get(tfsm_ctx *ctx, tfsm_ranker_weights *w) {
     if (!ctx->ranker_installed) return fail(ctx, TFSM_ERR_ARGUMENT, "no ranker installed");
     return guarded(ctx, [&]() -> int {
         std::vector<uint16_t> h(kParams);
-        HIPX(hipMemcpy(h.data(), ctx->d_rank_w, kParams * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        HIPX(hipMemcpyAsync(h.data(), ctx->d_rank_w, kParams * sizeof(uint16_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIPX(hipStreamSynchronize(ctx->stream));
         float *f = reinterpret_cast<float *>(w);
         for (int i = 0; i < kParams; ++i) f[i] = host_f32(h[i]);
         return TFSM_OK;
@@ -1362,15 +1366,23 @@ int tfsm_ranker_get(tfsm_ctx *ctx, tfsm_ranker_weights *w) {
 
 namespace {
 
-// Device copies of n feature rows and labels in the scratch arena.
+// Device copies of n feature rows and labels in the scratch arena.  An eval
+// call returns with work still queued on the context's stream that reads the
+// arena (the retained-domain copies and their descriptor list), so the
+// stream is drained before the arena is reused, and the copies go on that
+// stream.
 void stage_rows(tfsm_ctx *c, const float *x, const float *y, uint64_t n, float **dx, float **dy) {
+    sync_spin(c);
     c->work.reset();
     *dx = c->work.alloc<float>(n * kFeat);
-    HIPX(hipMemcpy(*dx, x, n * kFeat * sizeof(float), hipMemcpyHostToDevice));
+    HIPX(hipMemcpyAsync(*dx, x, n * kFeat * sizeof(float), hipMemcpyHostToDevice, c->stream));
     if (y) {
         *dy = c->work.alloc<float>(n);
-        HIPX(hipMemcpy(*dy, y, n * sizeof(float), hipMemcpyHostToDevice));
+        HIPX(hipMemcpyAsync(*dy, y, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     }
+    // the host buffers are caller-owned and pageable: the copies must finish
+    // before the call returns
+    HIPX(hipStreamSynchronize(c->stream));
 }
 
 }  // namespace
@@ -1429,11 +1441,14 @@ int tfsm_ranker_train(tfsm_ctx *ctx, const float *x, const float *y, uint64_t n,
         uint32_t *didx = ctx->work.alloc<uint32_t>(n);
         std::vector<uint16_t> hw(kParams);
         std::vector<float> hm(kParams);
-        HIPX(hipMemcpy(hw.data(), ctx->d_rank_w, kParams * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        // every copy and fill on the context's stream (a non-blocking stream
+        // does not wait for the null stream)
+        HIPX(hipMemcpyAsync(hw.data(), ctx->d_rank_w, kParams * sizeof(uint16_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIPX(hipStreamSynchronize(ctx->stream));
         for (int i = 0; i < kParams; ++i) hm[i] = host_f32(hw[i]);
-        HIPX(hipMemcpy(master, hm.data(), kParams * sizeof(float), hipMemcpyHostToDevice));
-        HIPX(hipMemset(m1, 0, kParams * sizeof(float)));
-        HIPX(hipMemset(m2, 0, kParams * sizeof(float)));
+        HIPX(hipMemcpyAsync(master, hm.data(), kParams * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        HIPX(hipMemsetAsync(m1, 0, kParams * sizeof(float), ctx->stream));
+        HIPX(hipMemsetAsync(m2, 0, kParams * sizeof(float), ctx->stream));
         std::vector<uint32_t> perm(n);
         for (uint64_t i = 0; i < n; ++i) perm[i] = (uint32_t)i;
         uint64_t state = seed ? seed : 0x9E3779B97F4A7C15ull;

@@ -66,7 +66,13 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
     stats = MinerStats{};
     found.clear();
     GrowthContext gc;
-    gc.max_nodes = opt.max_nodes;
+    // growth stops at the device's pattern limits rather than failing the run:
+    // at most TFSM_MAX_PATTERN_VERTICES vertices (forward edges), and children
+    // with more than TFSM_MAX_PATTERN_EDGES edges are dropped below
+    gc.max_nodes = opt.max_nodes < 0 || opt.max_nodes > TFSM_MAX_PATTERN_VERTICES ? TFSM_MAX_PATTERN_VERTICES
+                                                                                     : opt.max_nodes;
+    if (opt.max_nodes < 0 || opt.max_nodes > TFSM_MAX_PATTERN_VERTICES)
+        stats.capped_vertices = TFSM_MAX_PATTERN_VERTICES;
     for (int32_t l : g.elabel_values) gc.freq_edge_labels.insert(l);
     // size-2 level (grami.h:83-169): every surviving endpoint-set entry is frequent
     std::vector<Pattern> size2;
@@ -95,7 +101,19 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
     eo.task_order = opt.learned ? TFSM_ORDER_LEARNED : TFSM_ORDER_REFERENCE;
     eo.trace = opt.trace ? 1 : 0;
 
-    std::vector<uint64_t> live_handles;
+    // retained-domain handles of the previous level (parents of `level`) and
+    // of the level being evaluated: released on every exit, errors included
+    std::vector<uint64_t> live_handles, new_handles;
+    struct Release {
+        tfsm_ctx *ctx;
+        std::vector<uint64_t> *a, *b;
+        ~Release() {
+            for (auto *v : {a, b}) {
+                for (uint64_t h : *v) tfsm_domain_release(ctx, h);
+                v->clear();
+            }
+        }
+    } release{ctx, &live_handles, &new_handles};
     while (!level.empty()) {
         auto tl = std::chrono::steady_clock::now();
         const size_t n = level.size();
@@ -141,17 +159,21 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
                 account();
             }
             for (size_t k = 0; k < where.size(); ++k) res[where[k]] = subres[k];
+            for (auto &r : subres)
+                if (r.domain_handle) new_handles.push_back(r.domain_handle);
         } else {
             for (size_t i = 0; i < n; ++i) {
                 if (!ok[i]) continue;
                 int rc = tfsm_eval_pattern(ctx, &dp[i], parents[i], (uint32_t)opt.freq, &eo, &res[i]);
                 if (rc != TFSM_OK) { err = tfsm_last_error(ctx); return rc; }
+                if (res[i].domain_handle) new_handles.push_back(res[i].domain_handle);
                 account();
             }
         }
         // the parents' domains are no longer needed once all their children ran
         for (uint64_t h : live_handles) tfsm_domain_release(ctx, h);
-        live_handles.clear();
+        live_handles.swap(new_handles);
+        new_handles.clear();
 
         LevelStats ls;
         ls.size = level.empty() ? 0 : (uint32_t)level[0].p.nv();
@@ -163,7 +185,6 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
             stats.searches += res[i].searches;
             if (!res[i].frequent) continue;
             freq_idx.push_back(i);
-            if (res[i].domain_handle) live_handles.push_back(res[i].domain_handle);
         }
         ls.frequent = freq_idx.size();
         grower.wait();
@@ -175,14 +196,19 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
             f.support_lower = res[i].support_lower;
             f.support_upper = res[i].support_upper;
             found.push_back(std::move(f));
-            for (auto &c : kids[i]) next.push_back(Candidate{std::move(c), res[i].domain_handle});
+            for (auto &c : kids[i]) {
+                if (c.ne() > TFSM_MAX_PATTERN_EDGES) {
+                    stats.dropped_large++;
+                    continue;
+                }
+                next.push_back(Candidate{std::move(c), res[i].domain_handle});
+            }
         }
         stats.frequent += freq_idx.size();
         ls.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tl).count();
         stats.levels.push_back(ls);
         level.swap(next);
     }
-    for (uint64_t h : live_handles) tfsm_domain_release(ctx, h);
     std::stable_sort(found.begin(), found.end(), [](const FoundPattern &a, const FoundPattern &b) {
         if (a.p.nv() != b.p.nv()) return a.p.nv() < b.p.nv();
         return a.p.key() < b.p.key();

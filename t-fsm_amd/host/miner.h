@@ -57,6 +57,8 @@ struct MinerStats {
     double search_kernel_ms = 0;    // sum of search-kernel launch durations
     uint64_t bytes_table = 0, bytes_search = 0;   // algorithmic bytes
     std::vector<LevelStats> levels;
+    int capped_vertices = 0;        // nonzero: growth stopped at this many vertices (the device limit)
+    uint64_t dropped_large = 0;     // children over the device's edge limit, not evaluated
 };
 
 // Runs the whole mining job; `found` receives the frequent patterns in a

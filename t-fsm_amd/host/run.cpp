@@ -100,7 +100,13 @@ int main(int argc, char **argv) {
         return 1;
     }
     tfsm::write_results(found, out);
-    auto sec = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+    if (st.capped_vertices)
+        std::fprintf(stderr, "run: warning: pattern growth stops at %d vertices (the device's pattern limit)\n",
+                     st.capped_vertices);
+    if (st.dropped_large)
+        std::fprintf(stderr, "run: warning: %llu candidate patterns over %d edges were not evaluated\n",
+                     (unsigned long long)st.dropped_large, TFSM_MAX_PATTERN_EDGES);
+    auto sec =[](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
     std::printf("[TIME] Loading Graph Time: %.3f s\n", sec(t1, t2));
     std::printf("[TIME] Mining Time: %.3f s\n", sec(t2, t3));
     std::printf("[TIME] Total Elapsed Time: %.3f s\n", sec(t1, t3));

@@ -3,6 +3,9 @@ point include/tfsm_gpu.h declares (no compute is called)."""
 import ctypes
 import os
 import re
+import subprocess
+
+import pytest
 
 import tfsm
 from conftest import REPO
@@ -36,3 +39,34 @@ def test_host_library_exports():
     lib = tfsm.host_lib()
     for name in ["tfsmh_load", "tfsmh_mine", "tfsmh_extend", "tfsmh_is_min_code", "tfsmh_ctx_create"]:
         assert hasattr(lib, name)
+
+
+REFERENCE_PARALLEL = "/root/reference/parallel"
+
+
+@pytest.mark.skipif(not os.path.isdir(REFERENCE_PARALLEL), reason="the reference sources are not on this host")
+def test_reference_binding_compiles_and_links(tmp_path):
+    """t-fsm_amd/integration/gpu_eval.h, the binding a maintainer adds to the
+    reference's parallel/ tree, compiles against the reference's own headers
+    (graph.h's Graph, vertex_t, edge_t, Pattern and dfs_code_t) and links
+    against libtfsm_gpu.so.  Nothing is run (no GPU here)."""
+    tu = tmp_path / "bind.cpp"
+    tu.write_text(
+        '#include "graph.h"\n'
+        '#include "gpu_eval.h"\n'
+        "int main() {\n"
+        "    GpuGraph G; Graph g; Pattern p; bool f; uint64_t h;\n"
+        "    std::vector<Pattern *> ps{&p}; std::vector<uint64_t> par{0}, kids; std::vector<char> fr;\n"
+        "    if (gpu_graph_init(G, g) != TFSM_OK) return 1;\n"
+        "    gpu_eval(G, p, 0, 1, &f, &h);\n"
+        "    gpu_eval_level(G, ps, par, 1, fr, kids);\n"
+        "    gpu_graph_destroy(G);\n"
+        "    return 0;\n"
+        "}\n")
+    out = tmp_path / "bind"
+    r = subprocess.run(["g++", "-std=c++14", "-fopenmp", "-Wall", "-Werror=return-type",
+                        "-I", REFERENCE_PARALLEL, "-I", os.path.join(REPO, "include"),
+                        "-I", os.path.join(REPO, "t-fsm_amd", "integration"), str(tu), "-o", str(out),
+                        "-L", tfsm.BUILD, "-ltfsm_gpu", "-Wl,-rpath," + tfsm.BUILD],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]

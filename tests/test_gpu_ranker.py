@@ -76,6 +76,45 @@ def test_gradient_matches_reference(ctx):
     assert np.all(np.abs(g - ref_g) <= tol + 1e-4 * np.abs(ref_g))
 
 
+def test_ranker_call_after_retaining_level(workdir):
+    """A level call returns with its retained-domain copies still queued on the
+    stream and reading the scratch arena; a ranker call made at once reuses
+    that arena.  The retained domains must still equal the oracle's child
+    domains (C(u) minus proven-invalid candidates)."""
+    g = tfsm.Graph(workload_path("gnm2k", 1, workdir), 40)
+    c = tfsm.Context(g)
+    og = orc.OracleGraph(workload_path("gnm2k", 1, workdir), 40)
+    csr = g.csr()
+    vv = [int(x) for x in csr["vlabel_values"]]
+    ev = [int(x) for x in csr["elabel_values"]]
+    cases = []
+    for key in sorted(golden_keys("gnm2k_f40_m4")):
+        vl, es = tfsm.parse_key(key)
+        if len(vl) >= 3:
+            cases.append((vl, es))
+        if len(cases) >= 40:
+            break
+    pats = [tfsm.make_pattern([vv.index(l) for l in vl], [(a, b, ev.index(l)) for a, b, l in es])
+            for vl, es in cases]
+    c.ranker_set(random_weights(21))
+    X = feature_like(200000, 22)
+    for _ in range(2):
+        res = c.eval_level(pats, 40, mode=tfsm.MODE_EXACT, retain=True)
+        c.ranker_forward(X)                     # reuses the arena at once
+        checked = 0
+        for (vl, es), r in zip(cases, res):
+            if not r.domain_handle:
+                continue
+            ref = og.eval(vl, es, 40, mode=1)
+            for u in range(len(vl)):
+                assert np.array_equal(c.domain_read(r.domain_handle, u), ref.child_domain(u)), f"{vl} {es} {u}"
+            c.domain_release(r.domain_handle)
+            checked += 1
+        assert checked > 10
+    c.close()
+    g.close()
+
+
 def test_training_reduces_loss(ctx):
     rng = np.random.default_rng(9)
     X = feature_like(20000, 10)
