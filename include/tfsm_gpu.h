@@ -110,6 +110,9 @@ typedef struct {
                                    (0: tuned default, 0xffffffff: never) */
     uint32_t long_fanout;       /* records per offer (0: tuned default, at most 64) */
     uint32_t match_order;       /* TFSM_MATCH_* (0: adaptive) */
+    uint32_t group_lanes;       /* most lanes that share one root task when a search launch has
+                                   fewer tasks than resident lanes (0: tuned default, 1: one lane
+                                   per task, else a power of two <= 64); verdicts do not depend on it */
     uint32_t reserved;          /* must be 0 */
 } tfsm_eval_options;
 

@@ -149,6 +149,8 @@ struct SearchCounters {
     uint32_t pad6[31];
     uint32_t long_busy;                    // second-phase waves with a task or piece
     uint32_t pad7[31];
+    unsigned long long hot_next;           // next hot-tier entry
+    uint32_t pad8[30];
 };
 
 struct VerifyArgs {
@@ -180,6 +182,11 @@ struct VerifyArgs {
     uint32_t *long_list;                   // reference task indices for the wave-cooperative search
     uint32_t long_donate_every;            // its rounds between offers of work (0: never)
     uint32_t long_fanout;                  // records per offer (>= 1)
+    uint32_t group_max;                    // most lanes per root task (power of two; <= 1: one)
+    uint32_t group_base;                   // piece counter of this launch's entry 0 in a lane group
+    uint32_t group_cap;                    // entries a launch may run in lane groups
+    const uint32_t *hot;                   // with order: entries before *hot (of the first part)
+                                           // form the hot tier, searched with the largest groups
 };
 
 }  // namespace tfsm_dev

@@ -117,6 +117,9 @@ TFSM_LANE_FN uint32_t lane_gallop(const A &a, uint32_t lo, uint32_t hi, uint32_t
 template <int MV, int ORD = kOrderMixed>
 struct Lane {
     uint32_t u = 0, n = 0, d = 0, floor = 1;
+    // interleaved root search: of depth `floor`'s candidates this lane tests
+    // every stride-th one (the other lanes of its group take the others)
+    uint32_t stride = 1;
     uint32_t vis = 0;              // adaptive order: pattern vertices of depths 0 .. d
     bool adaptive_rt = false;      // the pattern's order (kOrderMixed)
     TFSM_LANE_FN bool adaptive() const { return ORD == kOrderAdaptive || (ORD == kOrderMixed && adaptive_rt); }
@@ -292,13 +295,19 @@ struct Lane {
         if (w != kNone) vis |= 1u << w;
     }
 
+    // A root task (u, ci).  With stride g > 1 and phase p < g, this lane
+    // tests only depth 1's candidates p, p + g, p + 2g, ... (the order of the
+    // depth-1 row and every deeper depth are as without the stride), so g
+    // lanes with phases 0 .. g-1 together search exactly the task's tree.
     template <class C, class A>
-    TFSM_LANE_FN void start_root(C &c, const A &a, uint32_t u_, uint32_t ci) {
+    TFSM_LANE_FN void start_root(C &c, const A &a, uint32_t u_, uint32_t ci, uint32_t stride_ = 1,
+                                 uint32_t phase = 0) {
         u = u_;
         n = a.nv();
         adaptive_rt = a.adaptive();
         floor = 1;
         d = 1;
+        stride = stride_;
         c.idx(u) = ci;
         if (adaptive()) {
             load_edges(a);
@@ -310,6 +319,9 @@ struct Lane {
             c.vid(0) = a.vid(u, ci);
             enter_reference(c, a, 1, kNone);
         }
+        // positions are below 2^32 - 64, so the shifted start cannot wrap;
+        // a start at or past the row's end is an exhausted share
+        c.pos(1) = c.pos(1) + phase;
     }
 
     // A piece: the candidates [lo, hi) of row `prim` at depth floor_, below
@@ -323,6 +335,7 @@ struct Lane {
         adaptive_rt = a.adaptive();
         floor = floor_;
         d = floor_;
+        stride = 1;
         if (adaptive()) {
             load_edges(a);
             vis = 0;
@@ -352,10 +365,12 @@ struct Lane {
         return adaptive() ? meta_vertex(c.meta(j)) : a.order(u, j);
     }
 
-    // Shallowest depth >= floor with untried candidates (d + 1 if none).
+    // Shallowest depth >= floor with untried candidates (d + 1 if none),
+    // whose untried candidates form the contiguous range [pos, end): an
+    // interleaved floor (stride > 1) is skipped.
     template <class C>
     TFSM_LANE_FN uint32_t open_depth(C &c) const {
-        uint32_t k = floor;
+        uint32_t k = stride > 1 ? floor + 1 : floor;
         while (k <= d && c.pos(k) >= c.end(k)) ++k;
         return k;
     }
@@ -370,7 +385,8 @@ struct Lane {
             --d;
             return d < floor ? kLaneExhausted : kLaneRunning;
         }
-        c.pos(d) = pos + 1;
+        const uint32_t adv = d == floor ? stride : 1u;
+        c.pos(d) = pos + adv;
         const uint32_t m = c.meta(d);
         const uint32_t prim = meta_prim(m);
         uint32_t w, nb, g0, sl;
@@ -388,9 +404,9 @@ struct Lane {
         const uint32_t cand = pos == pf_pos ? pf_val : a.pool(pos);
         // in flight beside this step's loads: the next primary entry and the
         // candidate's vertex id
-        if (pos + 1 < end) {
-            pf_pos = pos + 1;
-            pf_val = a.pool(pos + 1);
+        if (pos + adv < end) {
+            pf_pos = pos + adv;
+            pf_val = a.pool(pos + adv);
         }
         const uint32_t v = sl ? a.vid(w, cand) : 0u;
         for (uint32_t k = 0, j = 0; k < nb; ++k) {

@@ -28,6 +28,11 @@ using namespace tfsm_dev;
 constexpr uint32_t kSplitCap = 1u << 21;
 constexpr uint32_t kDefaultSplitSteps = 1024;
 constexpr uint32_t kDefaultSplitFanout = 1;
+// Lane groups: the default most lanes per root task, and the entries one
+// launch may run in groups (each has a piece counter beyond the records'
+// counters, two regions: one per search launch of a call).
+constexpr uint32_t kDefaultGroupLanes = 8;
+constexpr uint32_t kGroupCap = 1u << 16;
 // Wave-cooperative second phase: default step budget of a root task in the
 // first phase (off by default: on the power-law workloads its rounds, binary
 // searches in global memory, cost more than the lane-per-task steps they
@@ -455,6 +460,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     if (opt.task_order != TFSM_ORDER_REFERENCE && opt.task_order != TFSM_ORDER_LEARNED)
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown task order"};
     if (opt.reserved != 0) throw ArgError{TFSM_ERR_ARGUMENT, "reserved option field must be 0"};
+    if (opt.group_lanes > 64 || (opt.group_lanes & (opt.group_lanes - 1)))
+        throw ArgError{TFSM_ERR_ARGUMENT, "group_lanes must be 0 or a power of two up to 64"};
     if (opt.match_order != TFSM_MATCH_ADAPTIVE && opt.match_order != TFSM_MATCH_REFERENCE)
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown matching order"};
     const bool learned = opt.task_order == TFSM_ORDER_LEARNED;
@@ -680,7 +687,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     // work, the task bases) in one copy, the verdict bitmaps' fill in one
     // launch; the later kernels take the new descriptor copy
     PhaseBatch b2(c, n * sizeof(PatStatic) + aux_work.bytes() + 8 * task_base.size() + 3 * 256 +
-                         sizeof(CopyDesc) * (verdict_words / kFillPiece + 16) + 512);
+                         sizeof(CopyDesc) * (verdict_words / kFillPiece + 32) + 512);
     d_ps = b2.add(hs.data(), n);
     const WorkTable waux = aux_work.to_device(b2);
     const unsigned long long *d_task_base = b2.add(task_base);
@@ -694,6 +701,13 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         b2.zero(rank_sums, (size_t)n * kMaxV * (2 + kPrioQ) * sizeof(float));
     }
     if (task_base.back()) b2.zero(c->d_ctr, 2 * sizeof(SearchCounters));    // one per search launch
+    // the piece counters of lane-grouped entries start at zero, one region per search launch
+    const uint32_t group_max = opt.group_lanes == 0 ? kDefaultGroupLanes : opt.group_lanes;
+    if (task_base.back() && group_max > 1 && !opt.trace) {
+        const size_t gz = std::min<unsigned long long>(task_base.back(), kGroupCap);
+        b2.zero(c->d_piece_cnt + kSplitCap, gz * sizeof(uint32_t));
+        b2.zero(c->d_piece_cnt + kSplitCap + kGroupCap, gz * sizeof(uint32_t));
+    }
     b2.flush();
 
     // ---------------- auxiliary tables ----------------
@@ -745,8 +759,12 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         va.order = nullptr;
         va.order_pu = nullptr;
         va.split = nullptr;
+        va.hot = nullptr;
         va.part = 0;
         va.abort_stopped = 0;
+        va.group_max = group_max;
+        va.group_base = kSplitCap;
+        va.group_cap = kGroupCap;
         if (total_tasks >= 0xffffffffull) throw ArgError{TFSM_ERR_LIMIT, "more than 2^32 tasks in one call"};
         // features (traced calls and the learned order), scores, priority sort
         const uint32_t gt = grid_for(total_tasks, c->num_cus);
@@ -817,6 +835,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             // after the scan, the first deferred bin of block 0 holds the
             // number of first-part entries
             va.split = hist + (size_t)kDeferredKey * pg;
+            // and the first key after the hot tier, the hot tier's size
+            va.hot = hist + (size_t)kHotKeys * pg;
         }
         uint32_t epoch = ++c->epoch;
         HIPX(hipEventRecord(c->ev[4], s));
@@ -872,6 +892,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             epoch = ++c->epoch;
             va.ctr = c->d_ctr + 1;
             va.part = 2;
+            va.group_base = kSplitCap + kGroupCap;
             launch_verify();
         }
         verify_launches = two_parts ? 2u : 1u;
@@ -1118,10 +1139,14 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         HIPX(hipMalloc(&c->d_ctr, 3 * sizeof(SearchCounters)));
         c->d_flags = reinterpret_cast<uint32_t *>(c->d_ctr + 2);
         HIPX(hipMalloc(&c->d_records, (size_t)kSplitCap * sizeof(SplitRecord)));
-        HIPX(hipMalloc(&c->d_piece_cnt, (size_t)kSplitCap * sizeof(uint32_t)));
-        HIPX(hipMalloc(&c->d_piece_inc, (size_t)kSplitCap * sizeof(uint32_t)));
+        // piece counters: kSplitCap for records, then two regions of
+        // kGroupCap for lane-grouped root tasks
+        const size_t npieces = (size_t)kSplitCap + 2 * (size_t)kGroupCap;
+        HIPX(hipMalloc(&c->d_piece_cnt, npieces * sizeof(uint32_t)));
+        HIPX(hipMalloc(&c->d_piece_inc, npieces * sizeof(uint32_t)));
         HIPX(hipMemset(c->d_records, 0, (size_t)kSplitCap * sizeof(SplitRecord)));
-        HIPX(hipMemset(c->d_piece_inc, 0, (size_t)kSplitCap * sizeof(uint32_t)));
+        HIPX(hipMemset(c->d_piece_cnt, 0, npieces * sizeof(uint32_t)));
+        HIPX(hipMemset(c->d_piece_inc, 0, npieces * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_rank_w, kParams * sizeof(uint16_t)));
         return TFSM_OK;
     });

@@ -19,6 +19,7 @@ uint32_t *field(tfsm_eval_options &o, const std::string &name) {
     if (name == "long_waves_per_cu") return &o.long_waves_per_cu;
     if (name == "long_donate_every") return &o.long_donate_every;
     if (name == "long_fanout") return &o.long_fanout;
+    if (name == "group_lanes") return &o.group_lanes;
     if (name == "match_order") return &o.match_order;
     return nullptr;
 }

@@ -50,7 +50,7 @@ class EvalOptions(ctypes.Structure):
                 ("split_fanout", ctypes.c_uint32), ("long_steps", ctypes.c_uint32),
                 ("long_waves_per_cu", ctypes.c_uint32), ("long_donate_every", ctypes.c_uint32),
                 ("long_fanout", ctypes.c_uint32), ("match_order", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("group_lanes", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class PatternResult(ctypes.Structure):
@@ -439,7 +439,7 @@ def load_tuning(path):
     if host_lib().tfsmh_load_tuning(path.encode(), ctypes.byref(opt)) != 0:
         raise TfsmError(-1, host_lib().tfsmh_last_error().decode())
     names = ["grid_waves_per_cu", "split_steps", "split_fanout", "long_steps", "long_waves_per_cu",
-             "long_donate_every", "long_fanout", "match_order"]
+             "long_donate_every", "long_fanout", "match_order", "group_lanes"]
     return {k: getattr(opt, k) for k in names if getattr(opt, k)}
 
 

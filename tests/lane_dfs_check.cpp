@@ -21,6 +21,12 @@
 // entered in the same order):
 //
 //   lane_dfs_check --random SEED INSTANCES DONATE_EVERY FANOUT
+//
+// An optional last argument GROUP (a power of two, default 1) runs every
+// root task as GROUP interleaved lanes (phases 0 .. GROUP-1 of the depth-1
+// candidates, as the search kernel's lane groups do), each of which may
+// donate pieces; the task's verdict combines them all.  `crit_*` then sums,
+// over tasks, the most steps any one lane of the group took.
 #define TFSM_LANE_FN inline
 #include "../t-fsm_amd/csrc/lane_dfs.h"
 
@@ -184,7 +190,10 @@ struct Piece {
 struct TaskResult {
     bool found = false, all_exhausted = true;
     uint64_t steps = 0, pieces = 0;
+    uint64_t crit = 0;         // most steps of one root lane
 };
+
+uint32_t g_group = 1;          // lanes per root task
 
 // Runs task (u, ci) with pieces; `seq` (when set) collects the entered
 // (depth, candidate) pairs of the root piece.  ORD: the lane instance (the
@@ -254,11 +263,16 @@ TaskResult run_task_in(const HostAcc &a, uint32_t u, uint32_t ci, uint32_t donat
             }
         }
     };
-    c.poison();
-    L.start_root(c, a, u, ci);
-    uint32_t r = run(true);
-    res.found |= r == tfsm_lane::kLaneFound;
-    res.all_exhausted &= r == tfsm_lane::kLaneExhausted;
+    uint32_t r;
+    for (uint32_t phase = 0; phase < g_group; ++phase) {
+        c.poison();
+        L.start_root(c, a, u, ci, g_group, phase);
+        const uint64_t s0 = res.steps;
+        r = run(g_group == 1);
+        res.crit = std::max(res.crit, res.steps - s0);
+        res.found |= r == tfsm_lane::kLaneFound;
+        res.all_exhausted &= r == tfsm_lane::kLaneExhausted;
+    }
     for (size_t q = 0; q < queue.size(); ++q) {
         const Piece p = queue[q];
         if (p.prefix[0] != ci || p.pord[0] != u) violation("piece root", p.prefix[0], ci);
@@ -410,7 +424,7 @@ bool plain_exists(const HostAcc &a, uint32_t u, std::vector<uint32_t> &idx, uint
 
 int random_mode(uint64_t seed, uint32_t instances, uint32_t donate_every, uint32_t fanout) {
     Rng rng{seed};
-    uint64_t checked = 0, mismatches = 0, pieces = 0, found_total = 0, steps_ref = 0, steps_ad = 0;
+    uint64_t checked = 0, mismatches = 0, pieces = 0, found_total = 0, steps_ref = 0, steps_ad = 0, crit = 0;
     for (uint32_t it = 0; it < instances; ++it) {
         HostAcc a;
         std::vector<std::vector<int>> adj;
@@ -425,9 +439,10 @@ int random_mode(uint64_t seed, uint32_t instances, uint32_t donate_every, uint32
                 for (int adaptive = 0; adaptive < 2; ++adaptive) {
                     a.adaptive_ = adaptive != 0;
                     got_seq.clear();
-                    const bool visit = !donate_every && !adaptive;
+                    const bool visit = !donate_every && !adaptive && g_group == 1;
                     TaskResult r = run_task(a, u, ci, donate_every, fanout, a.edges, visit ? &got_seq : nullptr);
                     (adaptive ? steps_ad : steps_ref) += r.steps;
+                    crit += r.crit;
                     pieces += r.pieces;
                     found_total += r.found;
                     ++checked;
@@ -438,22 +453,29 @@ int random_mode(uint64_t seed, uint32_t instances, uint32_t donate_every, uint32
                 }
             }
     }
-    std::printf("random checked %llu mismatches %llu pieces %llu found %llu steps_reference %llu steps_adaptive %llu\n",
+    std::printf("random checked %llu mismatches %llu pieces %llu found %llu steps_reference %llu steps_adaptive %llu "
+                "crit %llu\n",
                 (unsigned long long)checked, (unsigned long long)mismatches, (unsigned long long)pieces,
-                (unsigned long long)found_total, (unsigned long long)steps_ref, (unsigned long long)steps_ad);
+                (unsigned long long)found_total, (unsigned long long)steps_ref, (unsigned long long)steps_ad,
+                (unsigned long long)crit);
     return mismatches ? 1 : 0;
 }
 
 }  // namespace
 
 int main(int argc, char **argv) {
-    if (argc == 6 && std::string(argv[1]) == "--random")
+    if ((argc == 6 || argc == 7) && std::string(argv[1]) == "--random") {
+        if (argc == 7) g_group = (uint32_t)std::atoi(argv[6]);
+        if (g_group < 1 || g_group > 64 || (g_group & (g_group - 1))) return 64;
         return random_mode(std::strtoull(argv[2], nullptr, 10), (uint32_t)std::atoi(argv[3]),
                            (uint32_t)std::atoi(argv[4]), (uint32_t)std::atoi(argv[5]));
-    if (argc != 4) {
-        std::fprintf(stderr, "usage: lane_dfs_check SPEC DONATE_EVERY FANOUT\n");
+    }
+    if (argc != 4 && argc != 5) {
+        std::fprintf(stderr, "usage: lane_dfs_check SPEC DONATE_EVERY FANOUT [GROUP]\n");
         return 64;
     }
+    if (argc == 5) g_group = (uint32_t)std::atoi(argv[4]);
+    if (g_group < 1 || g_group > 64 || (g_group & (g_group - 1))) return 64;
     const uint32_t donate_every = (uint32_t)std::atoi(argv[2]), fanout = (uint32_t)std::atoi(argv[3]);
     std::ifstream in(argv[1]);
     std::string graph_path;
@@ -467,7 +489,7 @@ int main(int argc, char **argv) {
     }
     std::string line;
     std::getline(in, line);
-    uint64_t checked = 0, mismatches = 0, patterns = 0, pieces = 0, steps_ref = 0, steps_ad = 0;
+    uint64_t checked = 0, mismatches = 0, patterns = 0, pieces = 0, steps_ref = 0, steps_ad = 0, crit = 0;
     while (std::getline(in, line)) {
         std::istringstream ls(line);
         int nv, ne;
@@ -530,6 +552,7 @@ int main(int argc, char **argv) {
                     a.adaptive_ = adaptive != 0;
                     TaskResult r = run_task(a, (uint32_t)u, (uint32_t)ci, donate_every, fanout, a.edges, nullptr);
                     (adaptive ? steps_ad : steps_ref) += r.steps;
+                    crit += r.crit;
                     pieces += r.pieces;
                     const uint8_t want = states[ci];
                     const uint8_t got = r.found ? 1 : r.all_exhausted ? 2 : 0;
@@ -542,8 +565,10 @@ int main(int argc, char **argv) {
         orc_eval_free(e);
     }
     orc_graph_free(g);
-    std::printf("patterns %llu checked %llu mismatches %llu pieces %llu steps_reference %llu steps_adaptive %llu\n",
+    std::printf("patterns %llu checked %llu mismatches %llu pieces %llu steps_reference %llu steps_adaptive %llu "
+                "crit %llu\n",
                 (unsigned long long)patterns, (unsigned long long)checked, (unsigned long long)mismatches,
-                (unsigned long long)pieces, (unsigned long long)steps_ref, (unsigned long long)steps_ad);
+                (unsigned long long)pieces, (unsigned long long)steps_ref, (unsigned long long)steps_ad,
+                (unsigned long long)crit);
     return mismatches ? 1 : 0;
 }

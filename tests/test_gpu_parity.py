@@ -209,6 +209,49 @@ def test_split_search_parity(split_steps, fanout, match, workdir):
         assert bool(cap.frequent) == ref.frequent
 
 
+@pytest.mark.parametrize("match", MATCH, ids=MATCH_IDS)
+@pytest.mark.parametrize("group,split_steps,fanout", [
+    (2, NEVER, 1),
+    (8, NEVER, 1),
+    (64, NEVER, 1),
+    (8, 16, 2),        # grouped lanes that also donate records
+    (1, 0, 0),         # one lane per task
+])
+def test_lane_group_parity(group, split_steps, fanout, match, workdir):
+    """Root tasks searched by lane groups (up to `group` lanes interleaving a
+    task's depth-1 candidates, the group's lanes pieces of one counter) give
+    the oracle's exact per-candidate verdicts and the capped verdicts, one
+    pattern per call and a whole level per call.  These calls have fewer
+    tasks than resident lanes, so the groups are used."""
+    env = env_for("gnm2k", 1, 40, workdir)
+    opts = dict(group_lanes=group, split_steps=split_steps, split_fanout=fanout, match_order=match)
+    cases = [c for c in variants(golden_keys("gnm2k_f40_m4"), 60) if env.dense(*c) is not None]
+    for vl, es in cases:
+        pat = env.dense(vl, es)
+        res = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_EXACT, **opts)
+        ref = env.og.eval(vl, es, env.freq, mode=1)
+        assert (res.support_lower, res.support_upper) == (ref.lower, ref.upper), f"{vl} {es}"
+        if ref.decided_by == 2:
+            for u in range(len(vl)):
+                assert np.array_equal(env.ctx.last_states(u), ref.states(u)), f"{vl} {es} verdicts {u}"
+        cap = env.ctx.eval_pattern(pat, env.freq, mode=tfsm.MODE_CAPPED, **opts)
+        assert bool(cap.frequent) == ref.frequent, f"{vl} {es}"
+    pats = [env.dense(vl, es) for vl, es in cases]
+    lev = env.ctx.eval_level(pats, env.freq, mode=tfsm.MODE_EXACT, **opts)
+    for (vl, es), r in zip(cases, lev):
+        ref = env.og.eval(vl, es, env.freq, mode=1)
+        assert (r.support_lower, r.support_upper) == (ref.lower, ref.upper), f"level {vl} {es}"
+
+
+def test_lane_group_option_checked(workdir):
+    env = env_for("gnm2k", 1, 40, workdir)
+    vl, es = sorted(variants(golden_keys("gnm2k_f40_m4"), 5))[0]
+    for bad in (3, 128):
+        with pytest.raises(tfsm.TfsmError) as e:
+            env.ctx.eval_pattern(env.dense(vl, es), env.freq, group_lanes=bad)
+        assert e.value.code == -1
+
+
 def test_adaptive_order_less_work(workdir):
     """The adaptive matching order gives the reference order's exact verdicts
     with fewer DFS steps in total (traced per task) on the power-law graph's

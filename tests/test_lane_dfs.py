@@ -36,13 +36,14 @@ def harness(tmp_path_factory):
 DONATION = [(0, 1), (1, 1), (2, 3)]     # (donate every N steps (0: never), pieces per donation)
 
 
-def run(harness, path, freq, samples, pats, tmp, donate=(0, 1)):
+def run(harness, path, freq, samples, pats, tmp, donate=(0, 1), group=1):
     spec = os.path.join(str(tmp), "spec.txt")
     with open(spec, "w") as f:
         f.write(f"{path} {freq} {samples}\n")
         for vl, es in pats:
             f.write(" ".join(str(x) for x in [len(vl)] + list(vl) + [len(es)] + [y for e in es for y in e]) + "\n")
-    r = subprocess.run([harness, spec, str(donate[0]), str(donate[1])], capture_output=True, text=True, timeout=600)
+    r = subprocess.run([harness, spec, str(donate[0]), str(donate[1]), str(group)], capture_output=True, text=True,
+                       timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     f = r.stdout.split()
     out = dict(zip(f[0::2], (int(x) for x in f[1::2])))
@@ -92,3 +93,28 @@ def test_random(harness, seed, instances, donate):
     out = dict(zip(f[1::2], (int(x) for x in f[2::2])))
     assert out["checked"] > 1000 and 0 < out["found"] < out["checked"], r.stdout
     assert (out["pieces"] > 0) == (donate[0] > 0), r.stdout
+
+
+@pytest.mark.parametrize("group,donate", [(2, (0, 1)), (4, (0, 1)), (8, (3, 2)), (64, (0, 1))])
+def test_lane_groups(harness, group, donate, workdir, tmp_path):
+    """Root tasks run as `group` interleaved lanes (lane p of the group tests
+    depth-1 candidates p, p + group, ...), each free to donate pieces: the
+    combined verdict equals the oracle's exact verdict, on G(2000, 8000) and
+    on sampled power-law candidates."""
+    out = run(harness, workload_path("gnm2k", 1, workdir), 40, 0, variants(golden_keys("gnm2k_f40_m4"), 120),
+              tmp_path, donate, group)
+    assert out["mismatches"] == 0 and (out["pieces"] > 0) == (donate[0] > 0), out
+    pats = [p for p in variants(golden_keys("powerlaw100k_f9000_m5"), 60) if len(p[0]) >= 4][:8]
+    out = run(harness, workload_path("powerlaw100k", 2, workdir), 9000, 20, pats, tmp_path, donate, group)
+    assert out["mismatches"] == 0, out
+
+
+@pytest.mark.parametrize("group", [2, 4, 16])
+def test_random_lane_groups(harness, group):
+    """Interleaved lane groups on synthetic tables against the plain recursive search."""
+    r = subprocess.run([harness, "--random", "5", "20", "0", "1", str(group)], capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    f = r.stdout.split()
+    out = dict(zip(f[1::2], (int(x) for x in f[2::2])))
+    assert out["checked"] > 1000 and out["mismatches"] == 0 and 0 < out["found"] < out["checked"], r.stdout
