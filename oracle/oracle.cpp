@@ -554,6 +554,13 @@ extern "C" void orc_eval_order(const orc_eval *e, int u, uint32_t *order) {
     for (int i = 0; i < e->nv; ++i) order[i] = e->order.empty() ? 0 : e->order[u][i];
 }
 
+extern "C" int orc_eval_bn(const orc_eval *e, int u, int d, uint32_t *out) {
+    if (e->bn.empty() || u < 0 || u >= e->nv || d < 0 || d >= e->nv) return -1;
+    const vector<int> &b = e->bn[u][d];
+    for (size_t k = 0; k < b.size(); ++k) out[k] = (uint32_t)b[k];
+    return (int)b.size();
+}
+
 // ---------------------------------------------------------------------------
 // Brute force: every injective, label- and edge-label-preserving map.
 // ---------------------------------------------------------------------------

@@ -71,6 +71,9 @@ uint32_t orc_eval_child_domain(const orc_eval *e, int u, uint32_t *out, uint32_t
 int64_t orc_eval_aux(const orc_eval *e, int a, int b, uint32_t *off, uint32_t *idx);
 /* GQL matching order starting at u (nv entries). */
 void orc_eval_order(const orc_eval *e, int u, uint32_t *order);
+/* Backward neighbours of depth d in the order starting at u (up to nv entries, in
+ * the pattern's adjacency order); returns their number, -1 without plans. */
+int orc_eval_bn(const orc_eval *e, int u, int d, uint32_t *out);
 
 /* Brute-force MNI: enumerates every injective label- and edge-label-preserving
  * map of the pattern into the pruned graph.  Fills images[u] counts; returns MNI.

@@ -55,6 +55,8 @@ def lib():
         L.orc_eval_aux.argtypes = [vp, ctypes.c_int, ctypes.c_int, u32p, u32p]
         L.orc_eval_aux.restype = ctypes.c_int64
         L.orc_eval_order.argtypes = [vp, ctypes.c_int, u32p]
+        L.orc_eval_bn.argtypes = [vp, ctypes.c_int, ctypes.c_int, u32p]
+        L.orc_eval_bn.restype = ctypes.c_int
         L.orc_bruteforce_mni.argtypes = [vp, ctypes.c_int, i32p, ctypes.c_int, i32p, u32p]
         L.orc_bruteforce_mni.restype = ctypes.c_uint32
         L.orc_mine.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
@@ -198,6 +200,11 @@ class OracleEval:
         out = np.zeros(self.nv, np.uint32)
         lib().orc_eval_order(self.h, u, _u32(out))
         return list(out)
+
+    def bn(self, u, d):
+        out = np.zeros(self.nv, np.uint32)
+        k = lib().orc_eval_bn(self.h, u, d, _u32(out))
+        return None if k < 0 else [int(x) for x in out[:k]]
 
     def close(self):
         if self.h:

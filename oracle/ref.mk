@@ -12,7 +12,7 @@ OUT     ?= oracle/_ref
 CXX     ?= g++
 CXXFLAGS = -std=c++14 -O3 -march=x86-64-v3 -fopenmp -w
 
-all: $(OUT)/run_parallel $(OUT)/run_serial
+all: $(OUT)/run_parallel $(OUT)/run_serial $(OUT)/ref_gmatch_dump
 
 $(OUT)/parallel_intersection.o: $(REF)/parallel/intersection/computesetintersection.cpp
 	@mkdir -p $(OUT)
@@ -36,7 +36,13 @@ $(OUT)/serial_run.o: $(REF)/serial/run.cpp $(wildcard $(REF)/serial/*.h)
 $(OUT)/run_serial: $(OUT)/serial_run.o $(OUT)/serial_intersection.o
 	$(CXX) $(CXXFLAGS) $^ -o $@
 
+# the fixture harness around the reference's matching engine
+# (tests/golden/ref_gmatch_dump.cpp, our one translation unit, includes parallel/gmatch.h)
+$(OUT)/ref_gmatch_dump: tests/golden/ref_gmatch_dump.cpp $(wildcard $(REF)/parallel/*.h) $(REF)/parallel/intersection/computesetintersection.cpp
+	@mkdir -p $(OUT)
+	$(CXX) $(CXXFLAGS) -I$(REF)/parallel -o $@ tests/golden/ref_gmatch_dump.cpp $(REF)/parallel/intersection/computesetintersection.cpp -lpthread
+
 clean:
-	rm -f $(OUT)/run_parallel $(OUT)/run_serial $(OUT)/*.o
+	rm -f $(OUT)/run_parallel $(OUT)/run_serial $(OUT)/ref_gmatch_dump $(OUT)/*.o
 
 .PHONY: all clean

@@ -1201,11 +1201,47 @@ int tfsm_eval_pattern(tfsm_ctx *ctx, const tfsm_pattern *p, uint64_t parent_hand
     return guarded(ctx, [&]() { return run_eval(ctx, EvalCall{p, &parent_handle, 1, freq, *opt, out}); });
 }
 
+// A level whose candidate bitmaps (two words per 32 data vertices per
+// pattern vertex) pass kLevelBitmapBudget bytes is evaluated in consecutive
+// slices of patterns, each one fused launch sequence: on a 1M-vertex graph a
+// level of 30K three-vertex candidates would otherwise need 25 GB of them.
+constexpr size_t kLevelBitmapBudget = size_t(1) << 30;
+
 int tfsm_eval_level(tfsm_ctx *ctx, const tfsm_pattern *ps, const uint64_t *parent_handles, uint32_t n, uint32_t freq,
                     const tfsm_eval_options *opt, tfsm_pattern_result *outs) {
     if (!ctx || !opt || (n && (!ps || !outs))) return TFSM_ERR_ARGUMENT;
     if (n == 0) return TFSM_OK;
-    return guarded(ctx, [&]() { return run_eval(ctx, EvalCall{ps, parent_handles, n, freq, *opt, outs}); });
+    return guarded(ctx, [&]() -> int {
+        for (uint32_t i = 0; i < n; ++i) check_pattern(ctx, ps[i]);      // no slice runs on bad input
+        const size_t per_vertex = 2 * (size_t)ctx->W * sizeof(uint32_t);
+        tfsm_call_stats sum{};
+        uint32_t slices = 0;
+        for (uint32_t i = 0; i < n;) {
+            size_t bytes = (size_t)ps[i].num_vertices * per_vertex;
+            uint32_t j = i + 1;
+            while (j < n && bytes + (size_t)ps[j].num_vertices * per_vertex <= kLevelBitmapBudget)
+                bytes += (size_t)ps[j++].num_vertices * per_vertex;
+            if (i == 0 && j < n && opt->trace)
+                throw ArgError{TFSM_ERR_LIMIT, "a traced level must fit one slice (split the level)"};
+            const int rc = run_eval(ctx, EvalCall{ps + i, parent_handles ? parent_handles + i : nullptr, j - i, freq,
+                                                  *opt, outs + i});
+            if (rc != TFSM_OK) return rc;
+            const tfsm_call_stats &s = ctx->stats;
+            sum.filter_ms += s.filter_ms;
+            sum.table_ms += s.table_ms;
+            sum.search_ms += s.search_ms;
+            sum.rank_ms += s.rank_ms;
+            sum.search_launches += s.search_launches;
+            sum.table_bytes += s.table_bytes;
+            sum.search_bytes += s.search_bytes;
+            sum.search_splits += s.search_splits;
+            sum.search_watchdog |= s.search_watchdog;
+            ++slices;
+            i = j;
+        }
+        if (slices > 1) ctx->stats = sum;
+        return TFSM_OK;
+    });
 }
 
 int tfsm_domain_read(tfsm_ctx *ctx, uint64_t handle, uint32_t u, uint32_t *out, uint32_t cap, uint32_t *size) {

@@ -199,3 +199,76 @@ def test_size2_statistics(seed, tmp_path):
         assert got == want
         assert og.n == sum(1 for l in labels if l in keep)
         assert og.num_adj == 2 * sum(1 for a, b, l in edges if labels[a] in keep and labels[b] in keep and l in ekeep)
+
+
+# ---------------------------------------------------------------------------
+# The reference's own matching engine (tests/golden/gmatch_*.txt, written by
+# tests/golden/make_gmatch_fixtures.py from parallel/gmatch.h compiled here):
+# filtered domains (DPisoFilter, gmatch.h:417-472), auxiliary tables
+# (buildTable, gmatch.h:995-1130), GQL orders and backward-neighbour lists
+# (gmatch.h:692-727, 803-828) must equal the oracle's.
+# ---------------------------------------------------------------------------
+def read_gmatch_fixture(path):
+    pats, cur = [], None
+    for line in open(path):
+        t = line.split()
+        if not t or t[0].startswith("#"):
+            continue
+        if t[0] == "P":
+            cur = dict(key=t[1], ok=None, dom={}, tab={}, order={}, bn={})
+            pats.append(cur)
+        elif t[0] == "F":
+            cur["ok"] = t[1] == "1"
+        elif t[0] == "D":
+            u, n = int(t[1]), int(t[2])
+            cur["dom"][u] = [int(x) for x in t[3:3 + n]]
+        elif t[0] == "T":
+            a, b, rows, ents = (int(x) for x in t[1:5])
+            vals = [int(x) for x in t[5:]]
+            cur["tab"][(a, b)] = (vals[:rows + 1], vals[rows + 1:rows + 1 + ents])
+        elif t[0] == "O":
+            cur["order"][int(t[1])] = [int(x) for x in t[2:]]
+        elif t[0] == "B":
+            cur["bn"][(int(t[1]), int(t[2]))] = [int(x) for x in t[3:]]
+    return pats
+
+
+def parse_key(key):
+    vl, es = {}, []
+    for code in key.strip(";").split(";"):
+        a, b, la, el, lb = (int(x) for x in code.split(","))
+        vl[a], vl[b] = la, lb
+        es.append((a, b, el))
+    return [vl[i] for i in range(len(vl))], es
+
+
+@pytest.mark.parametrize("name,workload,seed,freq", [("readme_f1", "readme", 0, 1), ("gnm2k_f40", "gnm2k", 1, 40)])
+def test_oracle_matches_reference_matcher(name, workload, seed, freq, workdir):
+    from conftest import GOLDEN, workload_path
+    pats = read_gmatch_fixture(os.path.join(GOLDEN, f"gmatch_{name}.txt"))
+    assert len(pats) >= 15
+    og = orc.OracleGraph(workload_path(workload, seed, workdir), freq)
+    searched = tabled = 0
+    for p in pats:
+        vl, es = parse_key(p["key"])
+        # support 1, as the fixture's DPisoFilter call: every pattern with
+        # non-empty domains goes on to tables and plans
+        r = og.eval(vl, es, 1, mode=1)
+        if not p["ok"]:
+            assert r.upper == 0, p["key"]          # some domain is empty
+            continue
+        for u, dom in p["dom"].items():
+            assert list(r.domain(u)) == dom, f"{p['key']} domain {u}"
+        if r.decided_by != 2:
+            continue                               # express path: no tables or plans
+        searched += 1
+        for (a, b), (off, idx) in p["tab"].items():
+            mine = r.aux(a, b)
+            assert mine is not None, f"{p['key']} table {a}->{b} missing"
+            assert list(mine[0]) == off and list(mine[1]) == idx, f"{p['key']} table {a}->{b}"
+            tabled += 1
+        for u, order in p["order"].items():
+            assert [int(x) for x in r.order(u)] == order, f"{p['key']} order {u}"
+        for (u, d), bn in p["bn"].items():
+            assert r.bn(u, d) == bn, f"{p['key']} bn {u} {d}"
+    assert searched >= 5 and tabled >= 10, (searched, tabled)
