@@ -22,6 +22,11 @@
 //
 //   lane_dfs_check --random SEED INSTANCES DONATE_EVERY FANOUT
 //
+// Level mode (every pattern of SPEC in one launch with the kernel's record
+// queue; REFERENCE 1 searches every pattern in the reference order):
+//
+//   lane_dfs_check --level SPEC SPLIT_STEPS FANOUT LANES SEED REFERENCE
+//
 // An optional last argument GROUP (a power of two, default 1) runs every
 // root task as GROUP interleaved lanes (phases 0 .. GROUP-1 of the depth-1
 // candidates, as the search kernel's lane groups do), each of which may
@@ -77,6 +82,10 @@ struct HostAcc {
     std::vector<int32_t> lab;
     std::vector<std::vector<uint32_t>> dom;                      // C(x), sorted data vertex ids
     std::vector<uint32_t> pool_;                                 // every table, back to back
+    // level mode: the tables of every pattern of the launch in one pool (as
+    // on the device, where positions are unique across a launch's patterns)
+    const std::vector<uint32_t> *shared = nullptr;
+    const std::vector<uint32_t> &pl() const { return shared ? *shared : pool_; }
     std::vector<std::vector<std::vector<uint32_t>>> off;         // off[b][x]: row starts over C(b), absolute
     std::vector<std::vector<uint32_t>> ord;                      // ord[u][d]
     std::vector<std::vector<std::vector<uint32_t>>> bn;          // bn[u][d]: backward neighbours of depth d
@@ -89,8 +98,8 @@ struct HostAcc {
     uint32_t edge_max() const { return (uint32_t)edges.size(); }
     bool adaptive() const { return adaptive_; }
     uint32_t pool(uint32_t pos) const {
-        if (pos >= pool_.size()) violation("pool", pos, pool_.size());
-        return pool_[pos];
+        if (pos >= pl().size()) violation("pool", pos, pl().size());
+        return pl()[pos];
     }
     uint32_t vid(uint32_t x, uint32_t c) const {
         if (x >= n || c >= dom[x].size()) violation("vid", c, x < n ? dom[x].size() : 0);
@@ -104,7 +113,7 @@ struct HostAcc {
         if ((size_t)r + 1 >= o.size()) violation("row index", r, o.size());
         lo = o[r];
         hi = o[r + 1];
-        if (lo > hi || hi > pool_.size()) violation("row range", hi, pool_.size());
+        if (lo > hi || hi > pl().size()) violation("row range", hi, pl().size());
     }
     uint32_t order(uint32_t u, uint32_t d) const {
         chk(u, d, "order");
@@ -422,6 +431,231 @@ bool plain_exists(const HostAcc &a, uint32_t u, std::vector<uint32_t> &idx, uint
     return false;
 }
 
+// Accessor over the oracle's domains, tables (appended to one pool) and
+// reference-order plans of one evaluated pattern.
+void build_acc(orc_eval *e, int nv, const std::vector<int32_t> &vl, int ne, const std::vector<int32_t> &ed,
+               HostAcc &a) {
+    a.n = (uint32_t)nv;
+    a.lab = vl;
+    a.dom.resize(nv);
+    for (int x = 0; x < nv; ++x) {
+        const uint32_t sz = orc_eval_domain(e, x, nullptr, 0);
+        a.dom[x].resize(sz);
+        orc_eval_domain(e, x, a.dom[x].data(), sz);
+    }
+    a.off.assign(nv, std::vector<std::vector<uint32_t>>(nv));
+    for (int b = 0; b < nv; ++b)
+        for (int x = 0; x < nv; ++x) {
+            const int64_t cnt = orc_eval_aux(e, b, x, nullptr, nullptr);
+            if (cnt < 0) continue;
+            std::vector<uint32_t> o(a.dom[b].size() + 1), ix((size_t)cnt);
+            orc_eval_aux(e, b, x, o.data(), ix.data());
+            const uint32_t base = (uint32_t)a.pool_.size();
+            a.pool_.insert(a.pool_.end(), ix.begin(), ix.end());
+            for (auto &v : o) v += base;
+            a.off[b][x] = std::move(o);
+        }
+    std::vector<std::vector<int>> adj(nv, std::vector<int>(nv, 0));
+    for (int i = 0; i < ne; ++i) {
+        adj[ed[3 * i]][ed[3 * i + 1]] = adj[ed[3 * i + 1]][ed[3 * i]] = 1;
+        a.edges.emplace_back((uint32_t)ed[3 * i], (uint32_t)ed[3 * i + 1]);
+    }
+    a.ord.assign(nv, std::vector<uint32_t>(nv));
+    a.bn.assign(nv, std::vector<std::vector<uint32_t>>(nv));
+    for (int u = 0; u < nv; ++u) {
+        orc_eval_order(e, u, a.ord[u].data());
+        for (int d = 0; d < nv; ++d)
+            for (int j = 0; j < d; ++j)
+                if (adj[a.ord[u][j]][a.ord[u][d]]) a.bn[u][d].push_back(a.ord[u][j]);
+    }
+}
+
+
+// Level mode: every pattern of the spec evaluated together as one launch of
+// the search kernel runs them, with its record queue: `lanes` virtual lanes
+// take root tasks (pattern-major, u ascending, candidates in order) while
+// any remain, then claim records from a FIFO; a piece donates the untried
+// candidates of its shallowest open depth, in up to `fanout` records of
+// consecutive ranges, once it has run split_steps steps since its last
+// donation, checked every 64 steps as in the kernel, and only when every
+// root task has been handed out and no record waits.  Each task has a piece
+// counter; its verdict is "valid" when some piece found an embedding and
+// "invalid" when the last piece ends without one.  The lanes run
+// `round` steps per turn in an order shuffled per round (seeded), so pieces
+// of different patterns interleave.  Verdicts must equal the oracle's.
+struct LevelTask {
+    uint32_t p, u, ci;
+    uint32_t pieces = 0;
+    bool found = false;
+    int verdict = 0;        // 0 pending, 1 valid, 2 invalid
+};
+
+struct LevelRecord {
+    uint32_t task, floor, prim, lo, hi;
+    uint32_t prefix[kMV];
+    uint8_t pord[kMV];
+};
+
+int level_mode(const char *spec_path, uint32_t split_steps, uint32_t fanout, uint32_t nlanes, uint64_t seed,
+               bool reference_order) {
+    std::ifstream in(spec_path);
+    std::string graph_path;
+    int freq = 0;
+    uint32_t samples = 0;
+    in >> graph_path >> freq >> samples;
+    orc_graph *g = orc_load(graph_path.c_str(), freq);
+    if (!g) return 64;
+    std::string line;
+    std::getline(in, line);
+    std::vector<HostAcc> accs;
+    std::vector<std::vector<std::vector<uint8_t>>> want;     // oracle states per pattern, vertex, candidate
+    std::vector<LevelTask> tasks;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        int nv, ne;
+        if (!(ls >> nv)) continue;
+        std::vector<int32_t> vl(nv);
+        for (auto &x : vl) ls >> x;
+        ls >> ne;
+        std::vector<int32_t> ed(3 * ne);
+        for (auto &x : ed) ls >> x;
+        if (nv > kMV) continue;
+        orc_eval *e = orc_eval_pattern(g, nv, vl.data(), ne, ed.data(), 0, nullptr, nullptr, freq, 1);
+        if (!e) continue;
+        if (orc_eval_decided_by(e) != 2) {
+            orc_eval_free(e);
+            continue;
+        }
+        HostAcc a;
+        build_acc(e, nv, vl, ne, ed, a);
+        a.adaptive_ = !reference_order && a.adaptive_fits();
+        const uint32_t p = (uint32_t)accs.size();
+        std::vector<std::vector<uint8_t>> st(nv);
+        for (int u = 0; u < nv; ++u) {
+            st[u].resize(a.dom[u].size());
+            orc_eval_states(e, u, st[u].data(), (uint32_t)st[u].size());
+            for (uint32_t ci = 0; ci < st[u].size(); ++ci) tasks.push_back(LevelTask{p, (uint32_t)u, ci});
+        }
+        accs.push_back(std::move(a));
+        want.push_back(std::move(st));
+        orc_eval_free(e);
+    }
+    orc_graph_free(g);
+    // one pool for the launch: each pattern's rows move to absolute positions
+    std::vector<uint32_t> pool;
+    for (HostAcc &a : accs) {
+        const uint32_t base = (uint32_t)pool.size();
+        pool.insert(pool.end(), a.pool_.begin(), a.pool_.end());
+        for (auto &ob : a.off)
+            for (auto &o : ob)
+                for (auto &v : o) v += base;
+        a.pool_.clear();
+    }
+    for (HostAcc &a : accs) a.shared = &pool;
+    struct VLane {
+        bool active = false;
+        uint32_t task = 0, steps = 0, since = 0;
+        HostCols c;
+        tfsm_lane::Lane<kMV, tfsm_lane::kOrderMixed> L;
+    };
+    std::vector<VLane> lanes(nlanes);
+    std::vector<LevelRecord> fifo;          // reserved records; head never passes the tail
+    size_t head = 0, next_task = 0;
+    uint64_t pieces = 0, records = 0, steps_total = 0;
+    Rng rng{seed};
+    std::vector<uint32_t> turn(nlanes);
+    for (uint32_t i = 0; i < nlanes; ++i) turn[i] = i;
+    auto end_piece = [&](VLane &ln, bool found) {
+        LevelTask &t = tasks[ln.task];
+        t.found |= found;
+        if (--t.pieces == 0) t.verdict = t.found ? 1 : 2;
+        ln.active = false;
+    };
+    const uint32_t round = 4;
+    for (;;) {
+        bool any = false;
+        for (uint32_t i = nlanes - 1; i > 0; --i) std::swap(turn[i], turn[rng.below(i + 1)]);
+        for (uint32_t li : turn) {
+            VLane &ln = lanes[li];
+            if (!ln.active) {
+                if (next_task < tasks.size()) {
+                    const LevelTask &t = tasks[next_task];
+                    ln.task = (uint32_t)next_task++;
+                    tasks[ln.task].pieces = 1;
+                    ln.c.poison();
+                    ln.c.n = accs[t.p].n;
+                    ln.L.start_root(ln.c, accs[t.p], t.u, t.ci);
+                } else if (head < fifo.size()) {
+                    const LevelRecord r = fifo[head++];
+                    const LevelTask &t = tasks[r.task];
+                    if (r.prefix[0] != t.ci || r.pord[0] != t.u) violation("record root", r.prefix[0], t.ci);
+                    ln.task = r.task;
+                    ln.c.poison();
+                    ln.c.n = accs[t.p].n;
+                    ln.L.start_piece(ln.c, accs[t.p], t.u, r.floor, r.prefix, r.pord, r.prim, r.lo, r.hi);
+                    ++pieces;
+                } else {
+                    continue;
+                }
+                ln.active = true;
+                ln.steps = ln.since = 0;
+            }
+            any = true;
+            const HostAcc &a = accs[tasks[ln.task].p];
+            for (uint32_t k = 0; k < round && ln.active; ++k) {
+                ++ln.steps;
+                ++ln.since;
+                if ((ln.steps & 63u) == 0 && split_steps && ln.since >= split_steps && next_task >= tasks.size() &&
+                    head >= fifo.size()) {
+                    ln.since = 0;
+                    const uint32_t kd = ln.L.open_depth(ln.c);
+                    if (kd <= ln.L.d) {
+                        const uint32_t lo = ln.c.pos(kd), rem = ln.c.end(kd) - lo;
+                        const uint32_t m = std::min(rem, fanout);
+                        tasks[ln.task].pieces += m;
+                        for (uint32_t q = 0; q < m; ++q) {
+                            LevelRecord r;
+                            r.task = ln.task;
+                            r.floor = kd;
+                            r.prim = tfsm_lane::meta_prim(ln.c.meta(kd));
+                            r.lo = lo + (uint32_t)((uint64_t)rem * q / m);
+                            r.hi = lo + (uint32_t)((uint64_t)rem * (q + 1) / m);
+                            for (uint32_t j = 0; j < kMV; ++j) {
+                                r.prefix[j] = 0xdeadbeefu;
+                                r.pord[j] = 0xee;
+                            }
+                            for (uint32_t j = 0; j <= kd && j < a.n; ++j) {
+                                const uint32_t w = ln.L.vertex_at(ln.c, a, j);
+                                r.pord[j] = (uint8_t)w;
+                                if (j < kd) r.prefix[j] = ln.c.idx(w);
+                            }
+                            fifo.push_back(r);
+                            ++records;
+                        }
+                        ln.c.end(kd) = lo;
+                    }
+                }
+                const uint32_t r = ln.L.step(ln.c, a);
+                ++steps_total;
+                if (r == tfsm_lane::kLaneFound) end_piece(ln, true);
+                else if (r == tfsm_lane::kLaneExhausted) end_piece(ln, false);
+            }
+        }
+        if (!any) break;
+    }
+    uint64_t mismatches = 0, valid = 0;
+    for (const LevelTask &t : tasks) {
+        const uint8_t w = want[t.p][t.u][t.ci];
+        valid += t.verdict == 1;
+        if (t.verdict != w && ++mismatches <= 10)
+            std::fprintf(stderr, "mismatch: pattern %u u %u ci %u got %d want %d\n", t.p, t.u, t.ci, t.verdict, w);
+    }
+    std::printf("level_patterns %zu tasks %zu valid %llu mismatches %llu records %llu pieces %llu steps %llu\n",
+                accs.size(), tasks.size(), (unsigned long long)valid, (unsigned long long)mismatches,
+                (unsigned long long)records, (unsigned long long)pieces, (unsigned long long)steps_total);
+    return mismatches || head != fifo.size() ? 1 : 0;
+}
+
 int random_mode(uint64_t seed, uint32_t instances, uint32_t donate_every, uint32_t fanout) {
     Rng rng{seed};
     uint64_t checked = 0, mismatches = 0, pieces = 0, found_total = 0, steps_ref = 0, steps_ad = 0, crit = 0;
@@ -464,6 +698,9 @@ int random_mode(uint64_t seed, uint32_t instances, uint32_t donate_every, uint32
 }  // namespace
 
 int main(int argc, char **argv) {
+    if (argc == 8 && std::string(argv[1]) == "--level")
+        return level_mode(argv[2], (uint32_t)std::atoi(argv[3]), (uint32_t)std::atoi(argv[4]),
+                          (uint32_t)std::atoi(argv[5]), std::strtoull(argv[6], nullptr, 10), std::atoi(argv[7]) != 0);
     if ((argc == 6 || argc == 7) && std::string(argv[1]) == "--random") {
         if (argc == 7) g_group = (uint32_t)std::atoi(argv[6]);
         if (g_group < 1 || g_group > 64 || (g_group & (g_group - 1))) return 64;
@@ -508,39 +745,7 @@ int main(int argc, char **argv) {
         }
         ++patterns;
         HostAcc a;
-        a.n = (uint32_t)nv;
-        a.lab = vl;
-        a.dom.resize(nv);
-        for (int x = 0; x < nv; ++x) {
-            const uint32_t sz = orc_eval_domain(e, x, nullptr, 0);
-            a.dom[x].resize(sz);
-            orc_eval_domain(e, x, a.dom[x].data(), sz);
-        }
-        a.off.assign(nv, std::vector<std::vector<uint32_t>>(nv));
-        for (int b = 0; b < nv; ++b)
-            for (int x = 0; x < nv; ++x) {
-                const int64_t cnt = orc_eval_aux(e, b, x, nullptr, nullptr);
-                if (cnt < 0) continue;
-                std::vector<uint32_t> o(a.dom[b].size() + 1), ix((size_t)cnt);
-                orc_eval_aux(e, b, x, o.data(), ix.data());
-                const uint32_t base = (uint32_t)a.pool_.size();
-                a.pool_.insert(a.pool_.end(), ix.begin(), ix.end());
-                for (auto &v : o) v += base;
-                a.off[b][x] = std::move(o);
-            }
-        std::vector<std::vector<int>> adj(nv, std::vector<int>(nv, 0));
-        for (int i = 0; i < ne; ++i) {
-            adj[ed[3 * i]][ed[3 * i + 1]] = adj[ed[3 * i + 1]][ed[3 * i]] = 1;
-            a.edges.emplace_back((uint32_t)ed[3 * i], (uint32_t)ed[3 * i + 1]);
-        }
-        a.ord.assign(nv, std::vector<uint32_t>(nv));
-        a.bn.assign(nv, std::vector<std::vector<uint32_t>>(nv));
-        for (int u = 0; u < nv; ++u) {
-            orc_eval_order(e, u, a.ord[u].data());
-            for (int d = 0; d < nv; ++d)
-                for (int j = 0; j < d; ++j)
-                    if (adj[a.ord[u][j]][a.ord[u][d]]) a.bn[u][d].push_back(a.ord[u][j]);
-        }
+        build_acc(e, nv, vl, ne, ed, a);
         const bool fits = a.adaptive_fits();
         for (int u = 0; u < nv; ++u) {
             std::vector<uint8_t> states(a.dom[u].size());

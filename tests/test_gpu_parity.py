@@ -307,12 +307,12 @@ def test_long_search_parity(long_opts, name, seed, freq, golden, limit, workdir)
         assert bool(cap.frequent) == ref.frequent
 
 
-@pytest.mark.skip(reason="an earlier form of this case (with work sharing inside a wave, since removed) made "
-                         "a GPU memory fault; it stays off the GPU until the exact-mode level path with "
-                         "split_steps=64 and fanout 4 is shown safe on the CPU")
 def test_split_level_parity(workdir):
     """A whole level in one launch with aggressive splitting: exact supports
-    and verdict counts equal the per-pattern evaluations without splitting."""
+    and verdict counts equal the per-pattern evaluations without splitting.
+    (An earlier in-wave work-sharing scheme, since removed, faulted on this
+    case; the record queue it runs now is checked on the CPU first by
+    tests/test_lane_dfs.py::test_level_record_queue.)"""
     env = env_for("powerlaw100k", 2, 9000, workdir)
     pats = []
     for vl, es in variants(golden_keys("powerlaw100k_f9000_m5"), 40):

@@ -118,3 +118,28 @@ def test_random_lane_groups(harness, group):
     f = r.stdout.split()
     out = dict(zip(f[1::2], (int(x) for x in f[2::2])))
     assert out["checked"] > 1000 and out["mismatches"] == 0 and 0 < out["found"] < out["checked"], r.stdout
+
+
+@pytest.mark.parametrize("reference_order", [0, 1], ids=["adaptive", "reference"])
+def test_level_record_queue(harness, reference_order, workdir, tmp_path):
+    """Several power-law patterns in one launch, as the GPU parity test of a
+    whole level with split_steps=64 and four records per donation runs them:
+    256 virtual lanes take root tasks of every pattern, then records from one
+    FIFO (donated every 64 steps once the root tasks are out and no record
+    waits), each record starting a piece of its own task's pattern from the
+    prefix it carries; per-task piece counters give the verdicts.  With
+    every access bounds-checked (and the sanitizers on), the verdicts equal
+    the oracle's exact ones and records are used."""
+    path = workload_path("powerlaw100k", 2, workdir)
+    pats = [p for p in variants(golden_keys("powerlaw100k_f9000_m5"), 40) if len(p[0]) >= 4][7:14]
+    spec = os.path.join(str(tmp_path), "level.spec")
+    with open(spec, "w") as f:
+        f.write(f"{path} 9000 0\n")
+        for vl, es in pats:
+            f.write(" ".join(str(x) for x in [len(vl)] + list(vl) + [len(es)] + [y for e in es for y in e]) + "\n")
+    r = subprocess.run([harness, "--level", spec, "64", "4", "256", "1", str(reference_order)], capture_output=True,
+                       text=True, timeout=900)
+    assert r.returncode == 0, r.stdout + r.stderr
+    f = r.stdout.split()
+    out = dict(zip(f[0::2], (int(x) for x in f[1::2])))
+    assert out["level_patterns"] >= 5 and out["mismatches"] == 0 and out["records"] > 0, r.stdout
