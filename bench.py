@@ -21,6 +21,7 @@ frequent); its reference run takes minutes, so it has no CPU baseline here.
 import argparse
 import json
 import os
+import statistics
 import subprocess
 import sys
 import tempfile
@@ -43,12 +44,30 @@ HBM_PEAK_GBPS = 8000.0
 RANKER = os.path.join(REPO, "t-fsm_amd", "rankers", "powerlaw.f32")
 
 
-def host_threads():
+def host_info():
+    """Host cores: what the OS shows, what this process may use, and the CPU
+    share this GPU is allotted (OMP_NUM_THREADS on a GPU box: 16 host cores
+    per GPU; worker pools there must stay within it)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        affinity = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or affinity
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": affinity, "share": share, "model": model,
+            "threads": max(1, min(affinity, share))}
+
+
+def host_threads():
+    return host_info()["threads"]
 
 
 def golden_keys(workload):
@@ -66,25 +85,49 @@ def read_keys(path, min_vertices=2):
     return keys if min_vertices <= 2 else set(k for k in keys if k.count(";") >= min_vertices - 1)
 
 
-def cpu_baseline(lg, threads):
-    """Reference parallel build on the host cores; one full run (about 5-10 s)."""
+def cpu_baseline(lg, threads, runs=3):
+    """The reference's parallel build on the host cores (`-thread` = the
+    threads this job may use), `runs` full runs of about 1.5-2 s each; its
+    [TIME] Mining Time (loading excluded) per run and their median."""
     ref = os.path.join(REPO, "oracle", "_ref", "run_parallel")
     if not os.path.exists(ref):
         return None
+    mining, walls, frequent = [], [], None
+    for _ in range(runs):
+        with tempfile.TemporaryDirectory() as tmp:
+            t0 = time.time()
+            out = subprocess.run([ref, "-file", lg, "-freq", str(WORKLOAD["freq"]), "-thread", str(threads),
+                                  "-maxNodes", str(WORKLOAD["max_nodes"])], cwd=tmp, capture_output=True, text=True,
+                                 timeout=600).stdout
+            walls.append(time.time() - t0)
+        for line in out.split("\n"):
+            if "Mining Time" in line:
+                mining.append(float(line.split(":")[1].split()[0]))
+            if "# Frequent Patterns" in line:
+                frequent = int(line.split(":")[1])
+    if len(mining) != runs:
+        return None
+    return {"mining_s": statistics.median(mining), "runs_s": [round(x, 3) for x in mining],
+            "wall_s": statistics.median(walls), "frequent": frequent}
+
+
+def oracle_baseline(lg, threads, runs=3):
+    """This build's own CPU oracle (oracle/, a multithreaded restatement of the
+    reference's algorithm) mining the same job on the same cores, capped
+    mode; median of `runs` runs (about 1-3 s each)."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    try:
+        import oracle as orc
+    except OSError:
+        return None
+    og = orc.OracleGraph(lg, WORKLOAD["freq"])
+    secs, n = [], 0
     with tempfile.TemporaryDirectory() as tmp:
-        t0 = time.time()
-        out = subprocess.run([ref, "-file", lg, "-freq", str(WORKLOAD["freq"]), "-thread", str(threads),
-                              "-maxNodes", str(WORKLOAD["max_nodes"])], cwd=tmp, capture_output=True, text=True,
-                             timeout=600).stdout
-        wall = time.time() - t0
-    info = {}
-    for line in out.split("\n"):
-        if "Mining Time" in line:
-            info["mining_s"] = float(line.split(":")[1].split()[0])
-        if "# Frequent Patterns" in line:
-            info["frequent"] = int(line.split(":")[1])
-    info["wall_s"] = wall
-    return info
+        for _ in range(runs):
+            n, st = og.mine(WORKLOAD["freq"], WORKLOAD["max_nodes"], threads, 0, os.path.join(tmp, "keys.txt"))
+            secs.append(st["mining_s"])
+    og.close()
+    return {"mining_s": statistics.median(secs), "runs_s": [round(x, 3) for x in secs], "frequent": n}
 
 
 def main():
@@ -175,16 +218,25 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
+        host = host_info()
         ref = cpu_baseline(lg, threads)
         if ref and ref.get("mining_s"):
+            orc = oracle_baseline(lg, threads)
             cpu = {"value": round(evaluated / ref["mining_s"], 3), "unit": "patterns/s", "cores": threads,
                    "kind": "reference",
-                   "sample": (f"one full run of the reference parallel build (oracle/_ref/run_parallel, -thread "
-                              f"{threads}) on the same graph; value = the {int(evaluated)} candidate patterns this "
-                              f"job evaluates / its [TIME] Mining Time {ref['mining_s']:.3f} s "
-                              f"(includes its 0.1 s idle-poll floor); it reported {ref.get('frequent')} frequent "
-                              f"patterns"),
-                   "mining_s": ref["mining_s"]}
+                   "sample": (f"the whole job, 3 full runs of the reference parallel build (oracle/_ref/run_parallel,"
+                              f" -thread {threads}) on the same graph; value = the {int(evaluated)} candidate "
+                              f"patterns this job evaluates / the median [TIME] Mining Time {ref['mining_s']:.3f} s "
+                              f"(it includes the reference's 0.1 s idle-poll floor); it reported "
+                              f"{ref.get('frequent')} frequent patterns. -thread is this GPU's host-core share "
+                              f"({host['share']}) of the {host['nproc']} cores the host shows"),
+                   "mining_s": ref["mining_s"], "runs_s": ref["runs_s"],
+                   "host": {"nproc": host["nproc"], "affinity": host["affinity"], "share": host["share"],
+                            "lscpu_model": host["model"]},
+                   "oracle": None if orc is None else {
+                       "value": round(evaluated / orc["mining_s"], 3), "mining_s": round(orc["mining_s"], 4),
+                       "runs_s": orc["runs_s"], "threads": threads, "frequent": orc["frequent"],
+                       "what": "this build's CPU oracle (oracle/oracle.cpp), capped mining of the same job"}}
 
     line = {
         "metric": "patterns mined/sec + MNI verifications/sec, batched+learned vs plain GPU",

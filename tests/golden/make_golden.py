@@ -19,6 +19,7 @@ import fcntl
 import hashlib
 import json
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -123,6 +124,10 @@ def main(names):
                 cmd = [os.path.join(REF, "run_serial"), "-file", lg, "-freq", str(freq), "-maxNodes", str(maxn)]
                 out = subprocess.run(cmd, cwd=tmp, check=True, capture_output=True, text=True).stdout
                 keys = parse_serial(os.path.join(tmp, "grami_results.txt"))
+                if workload in ("readme", "gnm2k"):
+                    # the reference's own output file, for the CLI output test
+                    shutil.copy(os.path.join(tmp, "grami_results.txt"),
+                                os.path.join(HERE, name + ".grami_results.txt"))
                 reported = int([l for l in out.split("\n") if "# Frequent Patterns" in l][0].split(":")[1])
             else:
                 cmd = [os.path.join(REF, "run_parallel"), "-file", lg, "-freq", str(freq), "-thread", str(threads),
