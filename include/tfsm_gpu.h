@@ -232,6 +232,20 @@ int tfsm_ranker_forward(tfsm_ctx *ctx, const float *x, uint64_t n, float *logits
  * respect to the installed weights. */
 int tfsm_ranker_gradient(tfsm_ctx *ctx, const float *x, const float *y, uint64_t n, tfsm_ranker_weights *grad,
                          float *loss);
+/* tfsm_ranker_gradient for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Inputs and outputs are DEVICE pointers
+ * owned by the caller: d_x (n x 24 rows), d_y (n labels), d_grad (the
+ * mean gradient, laid out as tfsm_ranker_weights: 1665 floats), d_loss (the
+ * mean loss; may be NULL).  It allocates nothing: the sums live in a fixed
+ * device array of TFSM_NOALLOC_SCRATCH_FLOATS floats (one per GPU, held
+ * until the process ends), so calls that overlap on one GPU race; each
+ * call clears the array first.  n is at most TFSM_NOALLOC_MAX_ROWS
+ * (TFSM_ERR_LIMIT beyond).  Returns when the results are written. */
+#define TFSM_NOALLOC_SCRATCH_FLOATS 1666
+#define TFSM_NOALLOC_MAX_ROWS (1ull << 31)
+int tfsm_ranker_gradient_noalloc(tfsm_ctx *ctx, const float *d_x, const float *d_y, uint64_t n, float *d_grad,
+                                 float *d_loss);
 /* Adam on the GPU from the installed weights (FP32 master copy, BF16 device
  * weights refreshed after each step); mini-batches drawn with a seeded
  * shuffle.  Installs the result; *loss gets the last epoch's mean loss. */

@@ -22,6 +22,30 @@
 
 using namespace tfsm_dev;
 
+// Scratch of tfsm_ranker_gradient_noalloc: the gradient and loss sums, one
+// copy per GPU for the process's life.  Host code never passes this name to
+// a kernel (in host code it is a stand-in, not a device address): each
+// context looks up its device's copy with hipGetSymbolAddress at creation.
+constexpr uint32_t kStaticGradWords = kParams + 1;
+static __device__ __align__(256) float scratch[kStaticGradWords];
+static_assert(kStaticGradWords == TFSM_NOALLOC_SCRATCH_FLOATS, "the header states the scratch size");
+
+namespace {
+
+__global__ __launch_bounds__(kBlock) void k_fill_f32(float *p, uint32_t n, float v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// out[i] = sums[i] * scale for the kParams gradient words, and the loss
+__global__ __launch_bounds__(kBlock) void k_scale_grad(const float *sums, float scale, float *grad_out,
+                                                       float *loss_out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t)kParams; i += gridDim.x * blockDim.x)
+        grad_out[i] = sums[i] * scale;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && loss_out) *loss_out = sums[kParams] * scale;
+}
+
+}  // namespace
+
 // Search-kernel work splitting: record capacity per launch, the default step
 // budget after which a piece donates work and the default number of records
 // a donation is cut into.
@@ -222,6 +246,7 @@ struct tfsm_ctx {
     uint16_t *d_rank_w = nullptr;            // installed BF16 parameters (kParams)
     bool ranker_installed = false;
     float *d_feat = nullptr;                 // features of the last traced call
+    float *d_static_grad = nullptr;          // this device's copy of `scratch`
     uint64_t feat_len = 0, feat_cap = 0;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     hipEvent_t ev_sync = nullptr;           // end of the stream's work, polled by sync_spin
@@ -1148,6 +1173,8 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         HIPX(hipMemset(c->d_piece_cnt, 0, npieces * sizeof(uint32_t)));
         HIPX(hipMemset(c->d_piece_inc, 0, npieces * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_rank_w, kParams * sizeof(uint16_t)));
+        // the no-allocation gradient's scratch: this device's copy of the array
+        HIPX(hipGetSymbolAddress(reinterpret_cast<void **>(&c->d_static_grad), HIP_SYMBOL(scratch)));
         return TFSM_OK;
     });
     if (rc != TFSM_OK) {
@@ -1460,6 +1487,28 @@ int tfsm_ranker_forward(tfsm_ctx *ctx, const float *x, uint64_t n, float *logits
                            ctx->d_rank_w, dl);
         HIPX(hipGetLastError());
         HIPX(hipMemcpyAsync(logits, dl, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIPX(hipStreamSynchronize(ctx->stream));
+        return TFSM_OK;
+    });
+}
+
+int tfsm_ranker_gradient_noalloc(tfsm_ctx *ctx, const float *d_x, const float *d_y, uint64_t n, float *d_grad,
+                                 float *d_loss) {
+    if (!ctx || !d_x || !d_y || !d_grad || !n) return TFSM_ERR_ARGUMENT;
+    if (!ctx->ranker_installed) return fail(ctx, TFSM_ERR_ARGUMENT, "no ranker installed");
+    if (n > TFSM_NOALLOC_MAX_ROWS)
+        return fail(ctx, TFSM_ERR_LIMIT, "tfsm_ranker_gradient_noalloc takes at most TFSM_NOALLOC_MAX_ROWS rows");
+    if (!ctx->d_static_grad) return fail(ctx, TFSM_ERR_HIP, "no scratch address for this device");
+    return guarded(ctx, [&]() -> int {
+        // the sums start at zero (a fill kernel: no memset node if a caller captures the stream)
+        float *sums = ctx->d_static_grad;
+        hipLaunchKernelGGL(k_fill_f32, dim3((kStaticGradWords + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream,
+                           sums, kStaticGradWords, 0.f);
+        hipLaunchKernelGGL(k_mlp_backward, dim3(grid_for(n, ctx->num_cus)), dim3(kBlock), 0, ctx->stream, d_x, d_y,
+                           (const uint32_t *)nullptr, n, ctx->d_rank_w, sums, sums + kParams);
+        hipLaunchKernelGGL(k_scale_grad, dim3((kParams + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, sums,
+                           1.f / (float)n, d_grad, d_loss);
+        HIPX(hipGetLastError());
         HIPX(hipStreamSynchronize(ctx->stream));
         return TFSM_OK;
     });

@@ -96,7 +96,8 @@ GPU_SYMBOLS = ["tfsm_ctx_create", "tfsm_ctx_destroy", "tfsm_ctx_set_pool_capacit
                "tfsm_domain_read", "tfsm_domain_release", "tfsm_last_domain", "tfsm_last_states",
                "tfsm_last_aux", "tfsm_last_order", "tfsm_read_trace", "tfsm_last_stats",
                "tfsm_status_string", "tfsm_last_error", "tfsm_ranker_set", "tfsm_ranker_get",
-               "tfsm_ranker_forward", "tfsm_ranker_gradient", "tfsm_ranker_train", "tfsm_read_features"]
+               "tfsm_ranker_forward", "tfsm_ranker_gradient", "tfsm_ranker_train", "tfsm_read_features",
+               "tfsm_ranker_gradient_noalloc"]
 
 _gpu = None
 _host = None
@@ -148,6 +149,8 @@ def gpu_lib():
         lib.tfsm_ranker_train.argtypes = [vp, fp, fp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.c_float, ctypes.c_uint64, fp]
         lib.tfsm_read_features.argtypes = [vp, fp, ctypes.c_uint64]
+        # device pointers (integers) for the no-allocation gradient
+        lib.tfsm_ranker_gradient_noalloc.argtypes = [vp, vp, vp, ctypes.c_uint64, vp, vp]
         lib.tfsm_read_features.restype = ctypes.c_int64
         _gpu = lib
     return _gpu
