@@ -91,6 +91,21 @@ enum {
     TFSM_MATCH_REFERENCE = 1    /* GQL order fixed per start vertex (gmatch.h:692-727, 803-828) */
 };
 
+/* Push-down pruning (the reference's sub-pattern cache, decompose.h:19-94 and
+ * comper.h:1582-1612): vertex `vertex` of pattern `pattern` of the call keeps
+ * only candidates that lie in the retained child domain (C minus proven-invalid
+ * candidates) of vertex `source_vertex` of the retained handle `handle`, a
+ * frequent sub-pattern that embeds into the pattern with source_vertex on
+ * vertex.  Sound: a candidate outside it has no embedding of the sub-pattern,
+ * so none of the pattern.  The reference keeps only the proven-invalid part. */
+typedef struct {
+    uint32_t pattern;           /* index in the call's pattern array */
+    uint32_t vertex;
+    uint64_t handle;            /* a retained domain handle (tfsm_domain_*) */
+    uint32_t source_vertex;
+    uint32_t reserved;          /* must be 0 */
+} tfsm_pushdown;
+
 typedef struct {
     uint32_t mode;              /* TFSM_MODE_* */
     uint32_t retain_domains;    /* keep a frequent pattern's child domains on the device */
@@ -114,6 +129,8 @@ typedef struct {
                                    fewer tasks than resident lanes (0: tuned default, 1: one lane
                                    per task, else a power of two <= 64); verdicts do not depend on it */
     uint32_t reserved;          /* must be 0 */
+    uint32_t num_pushdown;      /* entries of pushdown (0: none) */
+    const tfsm_pushdown *pushdown;
 } tfsm_eval_options;
 
 /* decided_by */

@@ -96,6 +96,15 @@ struct PatState {
     uint32_t aux_total[kMaxSlots];      // entries of each slot's table
 };
 
+// A push-down source: a retained child domain (sorted ids) and its bitmap
+// of proven-invalid entries.
+struct PushSrc {
+    const uint32_t *dom;
+    const uint32_t *invalid;
+    uint32_t n;
+    uint32_t pad;
+};
+
 // One device-to-device copy of `words` 32-bit words (a zero fill when src is null).
 struct CopyDesc {
     const uint32_t *src;

@@ -151,6 +151,7 @@ struct Retained {
     void *buf = nullptr;
     size_t cls = 0;
     uint32_t nv = 0;
+    uint32_t vlab[kMaxV];            // dense labels of the pattern's vertices
     uint32_t size[kMaxV];
     uint32_t *dom[kMaxV];           // device pointers into buf
     uint32_t *invalid[kMaxV];
@@ -603,6 +604,34 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     // the pattern states, then one word: the filter's last change flag
     PatState *d_st = c->work.alloc<PatState>(n + 1);
 
+    // push-down sources per (pattern, vertex): the retained child domains
+    // the candidates must lie in
+    std::vector<uint32_t> ex_off;
+    std::vector<PushSrc> ex;
+    if (opt.num_pushdown) {
+        if (!opt.pushdown) throw ArgError{TFSM_ERR_ARGUMENT, "num_pushdown without a pushdown array"};
+        std::vector<std::vector<PushSrc>> per((size_t)n * kMaxV);
+        for (uint32_t i = 0; i < opt.num_pushdown; ++i) {
+            const tfsm_pushdown &d = opt.pushdown[i];
+            if (d.reserved != 0) throw ArgError{TFSM_ERR_ARGUMENT, "reserved pushdown field must be 0"};
+            if (d.pattern >= n || d.vertex >= call.ps[d.pattern].num_vertices)
+                throw ArgError{TFSM_ERR_ARGUMENT, "pushdown entry names no vertex of the call"};
+            auto it = c->retained.find(d.handle);
+            if (it == c->retained.end()) throw ArgError{TFSM_ERR_HANDLE, "unknown pushdown domain handle"};
+            const Retained &R = it->second;
+            if (d.source_vertex >= R.nv) throw ArgError{TFSM_ERR_ARGUMENT, "pushdown source vertex out of range"};
+            if (call.ps[d.pattern].vlabel[d.vertex] != R.vlab[d.source_vertex])
+                throw ArgError{TFSM_ERR_ARGUMENT, "pushdown source vertex has another label"};
+            per[(size_t)d.pattern * kMaxV + d.vertex].push_back(
+                PushSrc{R.dom[d.source_vertex], R.invalid[d.source_vertex], R.size[d.source_vertex], 0});
+        }
+        ex_off.assign((size_t)n * kMaxV + 1, 0);
+        for (size_t k = 0; k < per.size(); ++k) {
+            ex_off[k + 1] = ex_off[k] + (uint32_t)per[k].size();
+            ex.insert(ex.end(), per[k].begin(), per[k].end());
+        }
+    }
+
     // ---------------- filter: LDF, DP-iso to the fixpoint, compaction ----------------
     HostWork src_work;
     std::vector<uint32_t> pair_p, pair_u;
@@ -613,11 +642,14 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             pair_u.push_back(u);
         }
     // this phase's uploads in one copy, its fills in one launch
-    PhaseBatch b1(c, n * sizeof(PatStatic) + src_work.bytes() + 2 * 256 + 8 * pair_p.size() +
+    PhaseBatch b1(c, n * sizeof(PatStatic) + src_work.bytes() + 2 * 256 + 8 * pair_p.size() + 4 * ex_off.size() +
+                         sizeof(PushSrc) * ex.size() + 512 +
                          sizeof(CopyDesc) * (bits_words / kFillPiece + (n + 1) * sizeof(PatState) / 4 / kFillPiece + 8) +
                          512);
     PatStatic *d_ps = b1.add(hs.data(), n);
     WorkTable wsrc = src_work.to_device(b1);
+    const uint32_t *d_ex_off = ex.empty() ? nullptr : b1.add(ex_off);
+    const PushSrc *d_ex = ex.empty() ? nullptr : b1.add(ex);
     const uint32_t *d_pair_p = b1.add(pair_p), *d_pair_u = b1.add(pair_u);
     b1.zero(bits_region, bits_words * sizeof(uint32_t));
     b1.zero(d_st, (n + 1) * sizeof(PatState));
@@ -629,7 +661,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     b1.zero(d_chg, 3 * (size_t)nchg * sizeof(uint32_t));
     b1.flush();
     const uint32_t gsrc = grid_for(wsrc.total, c->num_cus);
-    if (wsrc.total) hipLaunchKernelGGL(k_ldf, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc);
+    if (wsrc.total)
+        hipLaunchKernelGGL(k_ldf, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc, d_ex_off, (const PushSrc *)d_ex);
     HIPX(hipGetLastError());
     // DP-iso passes in batches without host round trips: a pass returns at
     // once when the one before it removed nothing (the fixpoint is reached);
@@ -1020,6 +1053,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             for (uint32_t u = 0; u < P.nv; ++u) words += P.dom_size[u] + (P.dom_size[u] + 31) / 32;
             Retained R;
             R.nv = P.nv;
+            for (uint32_t u = 0; u < P.nv; ++u) R.vlab[u] = P.vlab[u];
             R.buf = c->cache.get(std::max<size_t>(words, 1) * sizeof(uint32_t), &R.cls);
             uint32_t *b = static_cast<uint32_t *>(R.buf);
             for (uint32_t u = 0; u < P.nv; ++u) {
@@ -1250,8 +1284,22 @@ int tfsm_eval_level(tfsm_ctx *ctx, const tfsm_pattern *ps, const uint64_t *paren
                 bytes += (size_t)ps[j++].num_vertices * per_vertex;
             if (i == 0 && j < n && opt->trace)
                 throw ArgError{TFSM_ERR_LIMIT, "a traced level must fit one slice (split the level)"};
+            // the slice's push-down entries, renumbered to the slice's patterns
+            tfsm_eval_options so = *opt;
+            std::vector<tfsm_pushdown> sp;
+            if (i != 0 || j != n) {
+                if (opt->num_pushdown && !opt->pushdown)
+                    throw ArgError{TFSM_ERR_ARGUMENT, "num_pushdown without a pushdown array"};
+                for (uint32_t k = 0; k < opt->num_pushdown; ++k)
+                    if (opt->pushdown[k].pattern >= i && opt->pushdown[k].pattern < j) {
+                        sp.push_back(opt->pushdown[k]);
+                        sp.back().pattern -= i;
+                    }
+                so.num_pushdown = (uint32_t)sp.size();
+                so.pushdown = sp.data();
+            }
             const int rc = run_eval(ctx, EvalCall{ps + i, parent_handles ? parent_handles + i : nullptr, j - i, freq,
-                                                  *opt, outs + i});
+                                                  so, outs + i});
             if (rc != TFSM_OK) return rc;
             const tfsm_call_stats &s = ctx->stats;
             sum.filter_ms += s.filter_ms;

@@ -8,6 +8,7 @@
 #include "graph.h"
 #include "csr_cache.h"
 #include "miner.h"
+#include "pushdown.h"
 #include "pattern.h"
 #include "ranker.h"
 #include "tuning.h"
@@ -100,14 +101,16 @@ int tfsmh_ctx_create(const tfsmh_graph *h, tfsm_ctx **out) {
     return tfsm_ctx_create(&c, out);
 }
 
-// Runs the miner.  stats_out (15 doubles): mining_s, evaluated, frequent,
+// Runs the miner.  stats_out (16 doubles): mining_s, evaluated, frequent,
 // verdicts, searches, eval_calls, filter_ms, table_ms, search_ms,
-// search_launches, bytes_table, bytes_search, levels, max pattern size, 0.
+// search_launches, bytes_table, bytes_search, levels, max pattern size,
+// the sum of filtered domain sizes over the evaluated patterns, push-down
+// entries used.
 // results_path: grami_results.txt format; keys_path: one line per frequent
 // pattern "<dfs key>\t<support lower>\t<support upper>" (either may be NULL).
 // tuning (nullable): search-kernel shape for every evaluation call.
 int64_t tfsmh_mine(const tfsmh_graph *h, tfsm_ctx *ctx, int freq, int max_nodes, int threads, int batched, int mode,
-                   int learned, const char *results_path, const char *keys_path, double *stats_out,
+                   int learned, int pushdown, const char *results_path, const char *keys_path, double *stats_out,
                    const tfsm_eval_options *tuning) {
     tfsm::MinerOptions opt;
     if (tuning) opt.tuning = *tuning;
@@ -117,6 +120,7 @@ int64_t tfsmh_mine(const tfsmh_graph *h, tfsm_ctx *ctx, int freq, int max_nodes,
     opt.batched = batched != 0;
     opt.mode = mode;
     opt.learned = learned != 0;
+    opt.pushdown = pushdown != 0;
     std::vector<tfsm::FoundPattern> found;
     tfsm::MinerStats st;
     std::string err;
@@ -134,10 +138,11 @@ int64_t tfsmh_mine(const tfsmh_graph *h, tfsm_ctx *ctx, int freq, int max_nodes,
         }
     }
     if (stats_out) {
-        double v[15] = {st.mining_s, (double)st.evaluated, (double)st.frequent, (double)st.verdicts, (double)st.searches,
+        double v[16] = {st.mining_s, (double)st.evaluated, (double)st.frequent, (double)st.verdicts, (double)st.searches,
                         (double)st.eval_calls, st.gpu_ms[0], st.gpu_ms[1], st.gpu_ms[2], (double)st.search_launches,
                         (double)st.bytes_table, (double)st.bytes_search, (double)st.levels.size(),
-                        st.levels.empty() ? 0.0 : (double)st.levels.back().size, 0.0};
+                        st.levels.empty() ? 0.0 : (double)st.levels.back().size, (double)st.domain_sum,
+                        (double)st.pushdown_sources};
         std::memcpy(stats_out, v, sizeof(v));
     }
     return (int64_t)found.size();
@@ -168,6 +173,46 @@ int tfsmh_extend(const tfsmh_graph *h, int max_nodes, const int32_t *code, int n
             }
         }
         return (int)kids.size();
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// Canonical form of a small labelled graph (push-down cache keys): n
+// vertices with labels lab, edge labels el[a * n + b] (-1: none).  Writes the
+// key (NUL-terminated, cap bytes) and order[k] = the vertex at canonical
+// position k; returns the key length or -1.
+int tfsmh_canonical(int n, const int32_t *lab, const int32_t *el, char *key, int cap, int32_t *order) {
+    try {
+        if (n < 1 || n > tfsm::kMaxPatternVertices) return -1;
+        tfsm::Canon c = tfsm::canonical_form(n, lab, el);
+        if ((int)c.key.size() + 1 > cap) return -1;
+        std::memcpy(key, c.key.c_str(), c.key.size() + 1);
+        for (int k = 0; k < n; ++k) order[k] = c.order[k];
+        return (int)c.key.size();
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// The push-down decomposition of a pattern (5 ints per code edge): per
+// sub-pattern, its vertex count m then its m pattern vertices, back to back in
+// out (cap ints).  Returns the number of sub-patterns or -1.
+int tfsmh_decompose(const int32_t *code, int ne, int32_t *out, int cap) {
+    try {
+        tfsm::Pattern p;
+        for (int i = 0; i < ne; ++i)
+            p.push(tfsm::DfsCode{code[5 * i], code[5 * i + 1], code[5 * i + 2], code[5 * i + 3], code[5 * i + 4]});
+        int k = 0, cnt = 0;
+        for (const tfsm::SubPattern &s : tfsm::decompose(p)) {
+            if (k + 1 + (int)s.verts.size() > cap) return -1;
+            out[k++] = (int32_t)s.verts.size();
+            for (int v : s.verts) out[k++] = v;
+            ++cnt;
+        }
+        return cnt;
     } catch (const std::exception &e) {
         g_err = e.what();
         return -1;

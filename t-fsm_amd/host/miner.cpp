@@ -1,5 +1,6 @@
 // Level-synchronous miner; behaviour cited in miner.h.
 #include "miner.h"
+#include "pushdown.h"
 #include "worker_pool.h"
 
 #include <algorithm>
@@ -114,6 +115,11 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
             }
         }
     } release{ctx, &live_handles, &new_handles};
+    // push-down pruning: frequent patterns' retained domains stay in the cache
+    // (and alive) until the run ends
+    PushdownCache cache;
+    std::vector<uint64_t> cached_handles;
+    Release release_cached{ctx, &cached_handles, &cached_handles};
     while (!level.empty()) {
         auto tl = std::chrono::steady_clock::now();
         const size_t n = level.size();
@@ -126,6 +132,14 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
         }
         std::vector<tfsm_pattern_result> res(n);
         std::memset(res.data(), 0, n * sizeof(tfsm_pattern_result));
+        // push-down entries of each candidate from the cached frequent patterns
+        std::vector<std::vector<tfsm_pushdown>> push(n);
+        if (opt.pushdown && !cache.entries.empty()) {
+            parallel_for(n, opt.threads, [&](size_t i) {
+                if (ok[i]) push[i] = cache.lookup(level[i].p, 0, parents[i]);
+            });
+            for (auto &v : push) stats.pushdown_sources += v.size();
+        }
         // the next level's candidates are grown for every pattern of this
         // level while the GPU evaluates it (growth is pure: a pattern's
         // children depend on the pattern alone); only the frequent ones'
@@ -149,11 +163,23 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
             std::vector<tfsm_pattern> sub;
             std::vector<uint64_t> subpar;
             std::vector<size_t> where;
+            std::vector<tfsm_pushdown> subpush;
             for (size_t i = 0; i < n; ++i)
-                if (ok[i]) { sub.push_back(dp[i]); subpar.push_back(parents[i]); where.push_back(i); }
+                if (ok[i]) {
+                    for (tfsm_pushdown d : push[i]) {
+                        d.pattern = (uint32_t)sub.size();
+                        subpush.push_back(d);
+                    }
+                    sub.push_back(dp[i]);
+                    subpar.push_back(parents[i]);
+                    where.push_back(i);
+                }
             std::vector<tfsm_pattern_result> subres(sub.size());
             if (!sub.empty()) {
-                int rc = tfsm_eval_level(ctx, sub.data(), subpar.data(), (uint32_t)sub.size(), (uint32_t)opt.freq, &eo,
+                tfsm_eval_options lo = eo;
+                lo.num_pushdown = (uint32_t)subpush.size();
+                lo.pushdown = subpush.data();
+                int rc = tfsm_eval_level(ctx, sub.data(), subpar.data(), (uint32_t)sub.size(), (uint32_t)opt.freq, &lo,
                                          subres.data());
                 if (rc != TFSM_OK) { err = tfsm_last_error(ctx); return rc; }
                 account();
@@ -164,15 +190,25 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
         } else {
             for (size_t i = 0; i < n; ++i) {
                 if (!ok[i]) continue;
-                int rc = tfsm_eval_pattern(ctx, &dp[i], parents[i], (uint32_t)opt.freq, &eo, &res[i]);
+                tfsm_eval_options po = eo;
+                po.num_pushdown = (uint32_t)push[i].size();
+                po.pushdown = push[i].data();
+                int rc = tfsm_eval_pattern(ctx, &dp[i], parents[i], (uint32_t)opt.freq, &po, &res[i]);
                 if (rc != TFSM_OK) { err = tfsm_last_error(ctx); return rc; }
                 if (res[i].domain_handle) new_handles.push_back(res[i].domain_handle);
                 account();
             }
         }
         // the parents' domains are no longer needed once all their children ran
-        for (uint64_t h : live_handles) tfsm_domain_release(ctx, h);
-        live_handles.swap(new_handles);
+        // (with push-down pruning they stay cached until the run ends)
+        if (!opt.pushdown) {
+            for (uint64_t h : live_handles) tfsm_domain_release(ctx, h);
+            live_handles.swap(new_handles);
+        } else {
+            for (size_t i = 0; i < n; ++i)
+                if (res[i].frequent && res[i].domain_handle) cache.insert(level[i].p, res[i].domain_handle);
+            cached_handles.insert(cached_handles.end(), new_handles.begin(), new_handles.end());
+        }
         new_handles.clear();
 
         LevelStats ls;
@@ -181,6 +217,7 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
         std::vector<size_t> freq_idx;
         for (size_t i = 0; i < n; ++i) {
             stats.evaluated++;
+            for (int u = 0; u < level[i].p.nv(); ++u) stats.domain_sum += res[i].domain_size[u];
             stats.verdicts += res[i].verdicts;
             stats.searches += res[i].searches;
             if (!res[i].frequent) continue;

@@ -29,6 +29,7 @@ struct MinerOptions {
     int mode = TFSM_MODE_CAPPED;
     bool learned = false;      // learned task order (requires a ranker on the context)
     bool trace = false;        // keep per-task traces for ranker training
+    bool pushdown = false;     // push-down pruning from cached frequent sub-patterns (pushdown.h)
     // search-kernel shape (grid_waves_per_cu, split_*, long_*); mode, order,
     // trace and retention are set from the fields above
     tfsm_eval_options tuning{};
@@ -59,6 +60,8 @@ struct MinerStats {
     std::vector<LevelStats> levels;
     int capped_vertices = 0;        // nonzero: growth stopped at this many vertices (the device limit)
     uint64_t dropped_large = 0;     // children over the device's edge limit, not evaluated
+    uint64_t pushdown_sources = 0;  // push-down entries handed to the evaluation
+    uint64_t domain_sum = 0;        // filtered domain sizes summed over the evaluated patterns
 };
 
 // Runs the whole mining job; `found` receives the frequent patterns in a

@@ -3,7 +3,7 @@
 //   run -file G.lg -freq F [-thread T] [-maxNodes K]
 //       [-gpu plain|batched] [-support capped|exact] [-learned RANKER]
 //       [-tune TUNING.json] [-out grami_results.txt] [-stats stats.json]
-//       [-csrCache 1]
+//       [-csrCache 1] [-pushdown 1]
 //
 // Same flags and console summary as the reference (parallel/run.cpp:11-82);
 // flags are read by name in any order (the reference reads argv[2] as the
@@ -11,6 +11,8 @@
 // -tune a search-kernel shape written by the autotuner.  -csrCache 1 loads
 // the pruned graph from a binary cache beside the file (`<file>.f<freq>.csr`,
 // written on the first run and checked against the file's bytes).
+// -pushdown 1 turns on push-down pruning from cached frequent sub-patterns
+// (the reference's decompose.h / comper.h:1582-1612 cache).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -50,6 +52,7 @@ int main(int argc, char **argv) {
     if (const char *s = flag(argc, argv, "-maxNodes")) opt.max_nodes = std::atoi(s);
     if (const char *s = flag(argc, argv, "-gpu")) opt.batched = std::strcmp(s, "batched") == 0;
     if (const char *s = flag(argc, argv, "-support")) opt.mode = std::strcmp(s, "exact") == 0 ? TFSM_MODE_EXACT : TFSM_MODE_CAPPED;
+    if (const char *s = flag(argc, argv, "-pushdown")) opt.pushdown = std::strcmp(s, "0") != 0;
     const char *ranker_path = flag(argc, argv, "-learned");
     if (const char *t = flag(argc, argv, "-tune")) {
         std::string err;

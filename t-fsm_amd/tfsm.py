@@ -50,7 +50,8 @@ class EvalOptions(ctypes.Structure):
                 ("split_fanout", ctypes.c_uint32), ("long_steps", ctypes.c_uint32),
                 ("long_waves_per_cu", ctypes.c_uint32), ("long_donate_every", ctypes.c_uint32),
                 ("long_fanout", ctypes.c_uint32), ("match_order", ctypes.c_uint32),
-                ("group_lanes", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("group_lanes", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("num_pushdown", ctypes.c_uint32), ("pushdown", ctypes.c_void_p)]
 
 
 class PatternResult(ctypes.Structure):
@@ -174,7 +175,7 @@ def host_lib():
         lib.tfsmh_edge_stats.argtypes = [vp, P(ctypes.c_int32)]
         lib.tfsmh_ctx_create.argtypes = [vp, P(vp)]
         lib.tfsmh_mine.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                   ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, P(ctypes.c_double),
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, P(ctypes.c_double),
                                    P(EvalOptions)]
         lib.tfsmh_mine.restype = ctypes.c_int64
         lib.tfsmh_is_min_code.argtypes = [P(ctypes.c_int32), ctypes.c_int]
@@ -410,17 +411,19 @@ class Context:
         _check(self.h, self.lib.tfsm_domain_release(self.h, handle))
 
     def mine(self, freq, max_nodes=-1, threads=1, batched=False, mode=MODE_CAPPED, learned=False,
-             results_path=None, keys_path=None, **tuning):
-        """tuning: search-kernel shape for every evaluation (see eval_level)."""
-        stats = (ctypes.c_double * 15)()
+             results_path=None, keys_path=None, pushdown=False, **tuning):
+        """tuning: search-kernel shape for every evaluation (see eval_level);
+        pushdown: push-down pruning from cached frequent sub-patterns."""
+        stats = (ctypes.c_double * 16)()
         opt = EvalOptions(**tuning)
         n = host_lib().tfsmh_mine(self.graph.h, self.h, int(freq), int(max_nodes), int(threads), int(batched),
-                                  int(mode), int(learned), results_path.encode() if results_path else None,
+                                  int(mode), int(learned), int(pushdown), results_path.encode() if results_path else None,
                                   keys_path.encode() if keys_path else None, stats, ctypes.byref(opt))
         if n < 0:
             raise TfsmError(int(n), host_lib().tfsmh_last_error().decode())
         names = ["mining_s", "evaluated", "frequent", "verdicts", "searches", "eval_calls", "filter_ms", "table_ms",
-                 "search_ms", "search_launches", "bytes_table", "bytes_search", "levels", "max_size"]
+                 "search_ms", "search_launches", "bytes_table", "bytes_search", "levels", "max_size", "domain_sum",
+                 "pushdown_sources"]
         return int(n), dict(zip(names, list(stats)[:len(names)]))
 
     def close(self):
