@@ -74,7 +74,15 @@ typedef struct {
 
 enum {
     TFSM_MODE_CAPPED = 0,   /* reference semantics: stop at the frequent / infrequent verdict */
-    TFSM_MODE_EXACT = 1     /* decide every (u, v): exact MNI support */
+    TFSM_MODE_EXACT = 1,    /* decide every (u, v): exact MNI support */
+    TFSM_MODE_FRACTION = 2  /* Fraction-Score support (the reference's fraction-score build,
+                               fraction-score/graph.h:895-922, comper.h:478-508, 1082-1106):
+                               u's support is the sum over its valid candidates v of
+                               min over u's pattern neighbours' labels l of score(v, l)
+                               (tfsm_fraction_scores), the pattern's the minimum over u;
+                               frequent when it is at least freq.  Every (u, v) of a pattern
+                               is decided unless a vertex can no longer reach freq valid
+                               candidates (then the pattern is infrequent: a score is at most 1) */
 };
 
 enum {
@@ -147,6 +155,14 @@ typedef struct {
     uint64_t searches;          /* existence searches started */
     uint64_t verdicts;          /* final (u, v) verdicts, by search or by propagation */
     uint64_t domain_handle;     /* retained child domains (0: none) */
+    /* TFSM_MODE_FRACTION only (else 0): per u, the sum of the scores of its
+     * proven-valid candidates, exact for every pattern not rejected by the
+     * valid-count bound (so for every frequent one); fraction_support is
+     * their minimum.  Each sum is the double nearest the exact sum of the
+     * scores (ties to even): it does not depend on the order of the
+     * additions, which in the reference follows its threads. */
+    double fraction[TFSM_MAX_PATTERN_VERTICES];
+    double fraction_support;
 } tfsm_pattern_result;
 
 typedef struct tfsm_ctx tfsm_ctx;
@@ -175,6 +191,18 @@ int tfsm_eval_pattern(tfsm_ctx *ctx, const tfsm_pattern *p, uint64_t parent_hand
  * call that reads device memory from the host waits for the stream first. */
 int tfsm_eval_level(tfsm_ctx *ctx, const tfsm_pattern *ps, const uint64_t *parent_handles, uint32_t n,
                     uint32_t freq, const tfsm_eval_options *opt, tfsm_pattern_result *outs);
+
+/* Fraction scores of the graph's vertices (fraction-score/graph.h:895-922):
+ * out[v * num_vlabels + l] = score(v, l), from s = 0, for each neighbour w of
+ * v with label l in ascending id order (each adjacency entry once):
+ * s = min(1, s + 1 / NLF(w, label(v))), in IEEE double arithmetic in that
+ * order, so each value equals the reference's bit for bit; 0 where v has no
+ * neighbour with label l.  Computed on the device at the first call of this
+ * function or of a TFSM_MODE_FRACTION evaluation and kept with the context
+ * (8 * num_vertices * num_vlabels bytes).  Copies up to cap values and
+ * returns num_vertices * num_vlabels (negative: a tfsm_status).  The graph
+ * may have at most 64 edge labels (TFSM_ERR_LIMIT beyond). */
+int64_t tfsm_fraction_scores(tfsm_ctx *ctx, double *out, uint64_t cap);
 
 /* Child domain of pattern vertex u (C(u) minus proven-invalid candidates,
  * comper.h:1001-1018) of a retained handle; *size gets the full size. */

@@ -10,10 +10,12 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <functional>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <set>
@@ -34,8 +36,55 @@ struct orc_graph {
     vector<std::unordered_map<int32_t, int>> nlf;
     std::set<int32_t> freq_edge_labels;
     std::map<std::tuple<int32_t, int32_t, int32_t>, std::pair<std::set<uint32_t>, std::set<uint32_t>>> hashed_edges;
+    vector<std::map<int32_t, double>> vtx_frac;                // fraction-score/graph.h:895-922
     uint32_t n() const { return (uint32_t)vlabel.size(); }
+    double frac(uint32_t v, int32_t l) const {                // operator[] of the reference: 0 when absent
+        auto it = vtx_frac[v].find(l);
+        return it == vtx_frac[v].end() ? 0.0 : it->second;
+    }
 };
+
+namespace {
+
+// Correctly rounded sum of doubles: Shewchuk's exact partials ("Adaptive
+// Precision Floating-Point Arithmetic and Fast Robust Geometric Predicates",
+// 1997, GROW-EXPANSION with zero elimination), rounded once to nearest with
+// ties to even (the round-half-even step of Python's math.fsum).  The
+// Fraction-Score support of this build is the double nearest the exact sum.
+struct ExactSum {
+    vector<double> p;                                          // non-overlapping, increasing magnitude
+    void add(double x) {
+        size_t k = 0;
+        for (double y : p) {
+            if (std::fabs(x) < std::fabs(y)) std::swap(x, y);
+            const double hi = x + y, lo = y - (hi - x);
+            if (lo != 0.0) p[k++] = lo;
+            x = hi;
+        }
+        p.resize(k);
+        p.push_back(x);
+    }
+    double rounded() const {
+        if (p.empty()) return 0.0;
+        size_t n = p.size();
+        double hi = p[--n], lo = 0.0;
+        while (n > 0) {
+            const double x = hi, y = p[--n];
+            hi = x + y;
+            lo = y - (hi - x);
+            if (lo != 0.0) break;
+        }
+        // hi + lo is exact; a tie (lo exactly half an ulp) with more of the
+        // same sign below it rounds away from hi
+        if (n > 0 && ((lo < 0 && p[n - 1] < 0) || (lo > 0 && p[n - 1] > 0))) {
+            const double y = lo * 2, x = hi + y;
+            if (y == x - hi) hi = x;
+        }
+        return hi;
+    }
+};
+
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // Loading: read_input / construct_graph / find_frequent_labels /
@@ -118,7 +167,52 @@ extern "C" orc_graph *orc_load(const char *path, int freq) {
     g->nlf.resize(g->n());                                     // build_nlf, graph.h:850-866
     for (uint32_t i = 0; i < g->n(); ++i)
         for (auto &e : g->adj[i]) g->nlf[i][g->vlabel[e.first]] += 1;
+    // compute_fraction_score, fraction-score/graph.h:895-922: every edge
+    // entry (i, nbr) adds 1 / NLF(i, label(nbr)) to nbr's score for label(i),
+    // capped at 1 after each add, in the order i ascending, nbr ascending
+    g->vtx_frac.resize(g->n());
+    for (uint32_t i = 0; i < g->n(); ++i)
+        for (auto &e : g->adj[i]) {
+            double &s = g->vtx_frac[e.first][g->vlabel[i]];
+            s += 1 / (double)g->nlf[i][g->vlabel[e.first]];
+            if (s > 1) s = 1.0;
+        }
     return g;
+}
+
+extern "C" int64_t orc_fraction_scores(const orc_graph *g, double *out, uint64_t cap) {
+    const uint64_t L = g->nodes_by_label.size();
+    uint64_t k = 0;
+    for (uint32_t v = 0; v < g->n(); ++v)
+        for (auto &kv : g->nodes_by_label) {
+            if (out && k < cap) out[k] = g->frac(v, kv.first);
+            ++k;
+        }
+    return (int64_t)(L * g->n());
+}
+
+// The size-2 Fraction-Score filter, fraction-score/graph.h:803-824: over
+// each surviving endpoint set, sum_a = sum of score(v, to label) over candA,
+// sum_b = sum of score(v, from label) over candB.
+extern "C" int orc_edge_fraction(const orc_graph *g, double *out, int cap) {
+    int k = 0;
+    for (auto &kv : g->hashed_edges) {
+        if (k < cap) {
+            ExactSum a, b;
+            for (uint32_t v : kv.second.first) a.add(g->frac(v, std::get<2>(kv.first)));
+            for (uint32_t v : kv.second.second) b.add(g->frac(v, std::get<0>(kv.first)));
+            out[2 * k] = a.rounded();
+            out[2 * k + 1] = b.rounded();
+        }
+        ++k;
+    }
+    return k;
+}
+
+extern "C" double orc_exact_sum(const double *x, uint64_t n) {
+    ExactSum s;
+    for (uint64_t i = 0; i < n; ++i) s.add(x[i]);
+    return s.rounded();
 }
 
 extern "C" void orc_graph_free(orc_graph *g) { delete g; }
@@ -187,6 +281,7 @@ struct orc_eval {
     int frequent = 0, decided_by = 0;
     uint32_t lower = 0, upper = 0;
     uint64_t searches = 0, verdicts = 0;
+    vector<double> fraction;                                   // mode 2: per u, summed scores of valid candidates
 };
 
 namespace {
@@ -452,6 +547,14 @@ extern "C" orc_eval *orc_eval_pattern(const orc_graph *gp, int nv, const int32_t
     bool distinct = true;
     for (int a = 0; a < nv; ++a)
         for (int b = a + 1; b < nv; ++b) if (q.vlab[a] == q.vlab[b]) distinct = false;
+    // Fraction-Score term of candidate v of u (fraction-score/comper.h:486-499):
+    // min over u's pattern neighbours of v's score for the neighbour's label
+    auto frac_agg = [&](int u, uint32_t v) {
+        double f = std::numeric_limits<double>::max();
+        for (auto &e : q.adj[u]) f = std::min(f, g.frac(v, q.vlab[e.first]));
+        return f;
+    };
+    r->fraction.assign(nv, 0.0);
     if (distinct && ne == nv - 1) {
         // express path, comper.h:1619-1653: arc-consistent domains are exact
         r->decided_by = 1;
@@ -459,6 +562,17 @@ extern "C" orc_eval *orc_eval_pattern(const orc_graph *gp, int nv, const int32_t
         r->frequent = (int)min_size >= freq;
         for (auto &s : r->state) std::fill(s.begin(), s.end(), 1);
         for (auto &c : r->C) r->verdicts += c.size();
+        if (mode == 2) {
+            // fraction_score_is_freq, fraction-score/comper.h:1082-1106
+            double mn = std::numeric_limits<double>::max();
+            for (int u = 0; u < nv; ++u) {
+                ExactSum s;
+                for (uint32_t v : r->C[u]) s.add(frac_agg(u, v));
+                r->fraction[u] = s.rounded();
+                mn = std::min(mn, r->fraction[u]);
+            }
+            r->frequent = r->frequent && mn >= (double)freq;
+        }
         return r;
     }
     if ((int)min_size < freq) { r->decided_by = 0; return r; }          // isCandidateSetValid(freq)
@@ -469,7 +583,13 @@ extern "C" orc_eval *orc_eval_pattern(const orc_graph *gp, int nv, const int32_t
     for (int u = 0; u < nv; ++u) plan(g, q, r->C, u, r->order[u], r->bn[u]);
 
     // task processing: refill_Q order (u ascending, domain order), compute,
-    // refill_check and postprocess, comper.h:431-533, 1181-1300
+    // refill_check and postprocess, comper.h:431-533, 1181-1300.  Mode 2
+    // (the fraction-score build) keeps u's summed score beside its valid
+    // count: a vertex stops when the sum reaches freq (comper.h:478-508) and
+    // the pattern fails when the sum plus the unfinished candidates cannot
+    // (comper.h:299-301)
+    const bool early = mode == 0 || mode == 2;
+    vector<ExactSum> vsup(nv);
     vector<uint32_t> nvalid(nv, 0), ninvalid(nv, 0), finished(nv, 0);
     vector<bool> stops(nv, false);
     int done = 0;
@@ -477,7 +597,7 @@ extern "C" orc_eval *orc_eval_pattern(const orc_graph *gp, int nv, const int32_t
     vector<uint32_t> emb;
     for (int u = 0; u < nv && !decided; ++u) {
         for (uint32_t i = 0; i < r->C[u].size() && !decided; ++i) {
-            if (mode == 0 && stops[u]) break;
+            if (early && stops[u]) break;
             if (r->state[u][i] == 1) {                              // already valid through propagation
                 if (mode == 1) { r->verdicts++; continue; }
             }
@@ -487,7 +607,9 @@ extern "C" orc_eval *orc_eval_pattern(const orc_graph *gp, int nv, const int32_t
                     if (r->state[j][emb[j]] != 1) {
                         r->state[j][emb[j]] = 1;
                         nvalid[j]++;
-                        if (mode == 0 && !stops[j] && (int)nvalid[j] >= freq) {
+                        if (mode == 2) vsup[j].add(frac_agg(j, r->C[j][emb[j]]));
+                        const bool reached = mode == 2 ? vsup[j].rounded() >= (double)freq : (int)nvalid[j] >= freq;
+                        if (early && !stops[j] && reached) {
                             stops[j] = true;
                             if (++done == nv) { decided = true; frequent = true; }
                         }
@@ -503,6 +625,11 @@ extern "C" orc_eval *orc_eval_pattern(const orc_graph *gp, int nv, const int32_t
                 decided = true;                                     // domain_early_exit, comper.h:288-297
                 frequent = false;
             }
+            if (mode == 2 && !decided) {
+                ExactSum bound = vsup[u];
+                bound.add((double)((int64_t)r->C[u].size() - finished[u]));
+                if (bound.rounded() < (double)freq) { decided = true; frequent = false; }
+            }
         }
     }
     uint32_t lo = ~0u, hi = ~0u;
@@ -513,7 +640,25 @@ extern "C" orc_eval *orc_eval_pattern(const orc_graph *gp, int nv, const int32_t
     r->lower = lo;
     r->upper = hi;
     r->frequent = mode == 1 ? (int)lo >= freq : (decided ? frequent : (int)lo >= freq);
+    if (mode == 2) {
+        bool all = true;
+        for (int u = 0; u < nv; ++u) {
+            r->fraction[u] = vsup[u].rounded();
+            all = all && r->fraction[u] >= (double)freq;
+        }
+        r->frequent = decided ? frequent : all;
+    }
     return r;
+}
+
+extern "C" double orc_eval_fraction(const orc_eval *e, double *per_vertex) {
+    double mn = 0;
+    for (int u = 0; u < e->nv; ++u) {
+        const double f = u < (int)e->fraction.size() ? e->fraction[u] : 0.0;
+        if (per_vertex) per_vertex[u] = f;
+        mn = u == 0 ? f : std::min(mn, f);
+    }
+    return mn;
 }
 
 extern "C" void orc_eval_free(orc_eval *e) { delete e; }
@@ -631,16 +776,33 @@ extern "C" int64_t orc_mine(const orc_graph *gp, int freq, int max_nodes, int th
     int64_t nfreq = 0;
     uint64_t evaluated = 0, nverdicts = 0;
     vector<Item> level;
-    for (auto &kv : g.hashed_edges) {
+    // mode 2: the size-2 Fraction-Score filter (fraction-score/graph.h:803-824)
+    auto hashed = g.hashed_edges;
+    if (mode == 2)
+        for (auto it = hashed.begin(); it != hashed.end();) {
+            ExactSum a, b;
+            for (uint32_t v : it->second.first) a.add(g.frac(v, std::get<2>(it->first)));
+            for (uint32_t v : it->second.second) b.add(g.frac(v, std::get<0>(it->first)));
+            if (a.rounded() < (double)freq || b.rounded() < (double)freq) it = hashed.erase(it);
+            else ++it;
+        }
+    for (auto &kv : hashed) {
         int32_t la = std::get<0>(kv.first), el = std::get<1>(kv.first), lb = std::get<2>(kv.first);
         uint32_t s = (uint32_t)std::min(kv.second.first.size(), kv.second.second.size());
         ctx.neighbor_labels[la].insert(lb);
         ctx.neighbor_labels[lb].insert(la);
         tfsm::Pattern p = tfsm::single_edge(la, el, lb);
-        std::fprintf(out, "%s\t%u\t%u\n", p.key().c_str(), s, s);
+        std::fprintf(out, "%s\t%u\t%u", p.key().c_str(), s, s);
+        if (mode == 2) {
+            ExactSum a, b;
+            for (uint32_t v : kv.second.first) a.add(g.frac(v, lb));
+            for (uint32_t v : kv.second.second) b.add(g.frac(v, la));
+            std::fprintf(out, "\t%a", std::min(a.rounded(), b.rounded()));
+        }
+        std::fprintf(out, "\n");
         ++nfreq;
     }
-    for (auto &kv : g.hashed_edges) {
+    for (auto &kv : hashed) {
         tfsm::Pattern p = tfsm::single_edge(std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first));
         for (auto &c : tfsm::extend(p, ctx)) level.push_back(Item{std::move(c), -1});
     }
@@ -650,6 +812,7 @@ extern "C" int64_t orc_mine(const orc_graph *gp, int freq, int max_nodes, int th
         vector<char> freq_flag(level.size(), 0);
         vector<uint32_t> lo(level.size()), hi(level.size());
         vector<uint64_t> vd(level.size());
+        vector<double> fr(level.size());
         std::atomic<size_t> next{0};
         auto work = [&]() {
             size_t i;
@@ -663,6 +826,7 @@ extern "C" int64_t orc_mine(const orc_graph *gp, int freq, int max_nodes, int th
                                                par ? par->off.data() : nullptr, par ? par->dom.data() : nullptr, freq, mode);
                 freq_flag[i] = (char)e->frequent;
                 lo[i] = e->lower; hi[i] = e->upper; vd[i] = e->verdicts;
+                fr[i] = orc_eval_fraction(e, nullptr);
                 if (e->frequent) {
                     cur[i].off.push_back(0);
                     for (int u = 0; u < p.nv(); ++u) {
@@ -683,7 +847,9 @@ extern "C" int64_t orc_mine(const orc_graph *gp, int freq, int max_nodes, int th
             nverdicts += vd[i];
             if (!freq_flag[i]) continue;
             ++nfreq;
-            std::fprintf(out, "%s\t%u\t%u\n", level[i].p.key().c_str(), lo[i], hi[i]);
+            std::fprintf(out, "%s\t%u\t%u", level[i].p.key().c_str(), lo[i], hi[i]);
+            if (mode == 2) std::fprintf(out, "\t%a", fr[i]);
+            std::fprintf(out, "\n");
             for (auto &c : tfsm::extend(level[i].p, ctx)) nextlevel.push_back(Item{std::move(c), (int)i});
         }
         prev.swap(cur);

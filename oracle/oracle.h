@@ -18,6 +18,14 @@
  *                              forms (cliques, cycles, stars), reference
  *                              pattern sets through orc_mine
  *   orc_mine                 - the reference binaries' pattern sets
+ *   orc_fraction_scores      - bit for bit against the reference's own
+ *                              compute_fraction_score (a harness around
+ *                              fraction-score/graph.h), closed forms on stars
+ *   orc_edge_fraction,
+ *   orc_exact_sum            - Python's math.fsum on the same values; the
+ *                              reference harness's sums to within rounding
+ *   orc_eval / orc_mine (mode 2) - the reference fraction-score build's
+ *                              pattern sets
  */
 #ifndef TFSM_ORACLE_H
 #define TFSM_ORACLE_H
@@ -41,13 +49,23 @@ void orc_csr(const orc_graph *g, uint32_t *offsets, uint32_t *nbrs, int32_t *ela
 /* Surviving size-2 statistics: 5 int32 per entry (la, el, lb, |A|, |B|); returns count. */
 int orc_edge_stats(const orc_graph *g, int32_t *out, int cap);
 
+/* Fraction-Score table (fraction-score/graph.h:895-922): out[v * L + l], l the
+ * rank of the label among the pruned graph's labels (ascending); returns n * L. */
+int64_t orc_fraction_scores(const orc_graph *g, double *out, uint64_t cap);
+/* Size-2 Fraction-Score sums (fraction-score/graph.h:803-824) per surviving
+ * size-2 entry, in orc_edge_stats order: (sum over candA, sum over candB). */
+int orc_edge_fraction(const orc_graph *g, double *out, int cap);
+/* The double nearest the exact sum of x[0..n) (Shewchuk partials, ties to even). */
+double orc_exact_sum(const double *x, uint64_t n);
+
 /* Evaluate one pattern.
  *   nv, vlab[nv]           pattern vertex labels (original values)
  *   ne, edges[3*ne]        (a, b, edge label) per pattern edge
  *   parent_nv              number of leading pattern vertices whose candidates
  *                          start from the parent domains (0: none)
  *   parent_off[parent_nv+1], parent_dom[...]   parent domains (sorted ids)
- *   mode                   0 capped (reference early exits), 1 exact
+ *   mode                   0 capped (reference early exits), 1 exact,
+ *                          2 Fraction-Score (fraction-score/comper.h)
  */
 orc_eval *orc_eval_pattern(const orc_graph *g, int nv, const int32_t *vlab, int ne, const int32_t *edges,
                            int parent_nv, const uint32_t *parent_off, const uint32_t *parent_dom,
@@ -60,6 +78,9 @@ uint32_t orc_eval_support_lower(const orc_eval *e);
 uint32_t orc_eval_support_upper(const orc_eval *e);
 uint64_t orc_eval_searches(const orc_eval *e);
 uint64_t orc_eval_verdicts(const orc_eval *e);
+/* Mode 2: per u, the summed scores of the candidates found valid (all of
+ * them when the search ran to the end); returns the minimum over u. */
+double orc_eval_fraction(const orc_eval *e, double *per_vertex);
 /* Filtered candidate domain of u (after DP-iso); returns its size, copies up to cap ids. */
 uint32_t orc_eval_domain(const orc_eval *e, int u, uint32_t *out, uint32_t cap);
 /* Per-candidate state of u: 0 undecided, 1 valid, 2 invalid; returns |C(u)|. */

@@ -62,6 +62,14 @@ def lib():
         L.orc_mine.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
                                P(ctypes.c_double), P(ctypes.c_uint64), P(ctypes.c_uint64)]
         L.orc_mine.restype = ctypes.c_int64
+        dp = P(ctypes.c_double)
+        L.orc_fraction_scores.argtypes = [vp, dp, ctypes.c_uint64]
+        L.orc_fraction_scores.restype = ctypes.c_int64
+        L.orc_edge_fraction.argtypes = [vp, dp, ctypes.c_int]
+        L.orc_exact_sum.argtypes = [dp, ctypes.c_uint64]
+        L.orc_exact_sum.restype = ctypes.c_double
+        L.orc_eval_fraction.argtypes = [vp, dp]
+        L.orc_eval_fraction.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -72,6 +80,12 @@ def _u32(a):
 
 def _i32(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def exact_sum(values):
+    """The double nearest the exact sum (the oracle's Shewchuk partials)."""
+    a = np.ascontiguousarray(values, np.float64)
+    return lib().orc_exact_sum(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(a))
 
 
 class OracleGraph:
@@ -96,6 +110,20 @@ class OracleGraph:
         out = np.zeros(max(5 * k, 1), np.int32)
         lib().orc_edge_stats(self.h, _i32(out), k)
         return out[:5 * k].reshape(-1, 5)
+
+    def fraction_scores(self):
+        """(n, L) float64: score(v, l), labels in ascending order."""
+        k = lib().orc_fraction_scores(self.h, None, 0)
+        out = np.zeros(max(k, 1), np.float64)
+        lib().orc_fraction_scores(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), k)
+        return out[:k].reshape(self.n, -1) if self.n else out[:0].reshape(0, 0)
+
+    def edge_fraction(self):
+        """(k, 2) float64: the size-2 sums over candA and candB, in edge_stats order."""
+        k = lib().orc_edge_fraction(self.h, None, 0)
+        out = np.zeros(max(2 * k, 1), np.float64)
+        lib().orc_edge_fraction(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), k)
+        return out[:2 * k].reshape(-1, 2)
 
     def eval(self, vlabels, edges, freq, mode=0, parent_domains=None):
         """vlabels/edges in original label values; parent_domains: list of sorted id arrays."""
@@ -167,6 +195,12 @@ class OracleEval:
     @property
     def verdicts(self):
         return int(lib().orc_eval_verdicts(self.h))
+
+    def fraction(self):
+        """Mode 2: (per-vertex summed scores, their minimum)."""
+        out = np.zeros(self.nv, np.float64)
+        mn = lib().orc_eval_fraction(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out, mn
 
     def domain(self, u):
         n = lib().orc_eval_domain(self.h, u, None, 0)

@@ -41,6 +41,21 @@ struct DevGraph {
     const uint32_t *nbr;     // partitioned adjacency
 };
 
+// The early exits a support mode allows the existence search: the infrequent
+// verdict (some vertex can no longer reach freq valid candidates) in capped
+// and Fraction-Score mode (a score is at most 1, so a vertex's summed score is
+// at most its valid count), the frequent verdict (every vertex reached freq
+// valid candidates) in capped mode only.
+__host__ __device__ inline bool mode_stops_infrequent(uint32_t mode) { return mode != TFSM_MODE_EXACT; }
+__host__ __device__ inline bool mode_stops_frequent(uint32_t mode) { return mode == TFSM_MODE_CAPPED; }
+
+// Fraction-Score sum of one (pattern, vertex): the exact sum of the scores
+// of its valid candidates in units of 2^-96, as a 128-bit integer in two
+// words (every score is a double in [2^-32, 1], a multiple of 2^-84).
+struct FracSum {
+    unsigned long long lo, hi;
+};
+
 // Host-written description of one pattern in one eval call.
 struct PatStatic {
     uint32_t nv, nslots, freq, mode;

@@ -217,6 +217,7 @@ struct tfsm_ctx {
     uint32_t n = 0, L = 0, EL = 0, K = 0, W = 0;
     uint64_t nadj = 0;
     uint32_t *d_seg = nullptr, *d_nbr = nullptr, *d_bucket = nullptr;
+    double *d_score = nullptr;               // Fraction-Score table, n x L (computed on first use)
     std::vector<uint32_t> bucket_off;        // host copy, L + 1
     DevGraph g{};
     // scratch
@@ -232,6 +233,7 @@ struct tfsm_ctx {
     uint32_t epoch = 0;
     Pinned<PatState> h_state;
     Pinned<SearchCounters> h_ctr;
+    Pinned<FracSum> h_frac;
     // retained domains
     BufferCache cache;
     std::unordered_map<uint64_t, Retained> retained;
@@ -478,11 +480,46 @@ void sync_spin(tfsm_ctx *c) {
     HIPX(e);
 }
 
+// The Fraction-Score table (k_vertex_frac), computed once per context.
+void ensure_scores(tfsm_ctx *c) {
+    if (c->d_score) return;
+    if (c->EL > kFracMaxEL) throw ArgError{TFSM_ERR_LIMIT, "fraction scores support at most 64 edge labels"};
+    const unsigned long long cells = (unsigned long long)c->n * c->L;
+    double *d = nullptr;
+    uint32_t *d_off = nullptr;
+    HIPX(hipMalloc(&d, std::max<unsigned long long>(cells, 1) * sizeof(double)));
+    try {
+        if (cells) {
+            HIPX(hipMalloc(&d_off, c->bucket_off.size() * sizeof(uint32_t)));
+            HIPX(hipMemcpyAsync(d_off, c->bucket_off.data(), c->bucket_off.size() * sizeof(uint32_t),
+                                hipMemcpyHostToDevice, c->stream));
+            hipLaunchKernelGGL(k_vertex_frac, dim3(grid_for(cells, c->num_cus)), dim3(kBlock), 0, c->stream, c->g,
+                               c->d_bucket, d_off, c->L, d);
+            HIPX(hipGetLastError());
+            HIPX(hipStreamSynchronize(c->stream));
+            HIPX(hipFree(d_off));
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(d_off);
+        (void)hipFree(d);
+        throw;
+    }
+    c->d_score = d;
+}
+
+// The double nearest an exact Fraction-Score sum (ties to even).
+double frac_value(const FracSum &f) {
+    const unsigned __int128 x = ((unsigned __int128)f.hi << 64) | f.lo;
+    return std::ldexp((double)x, -96);
+}
+
 int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     const uint32_t n = call.n;
     const tfsm_eval_options &opt = call.opt;
-    if (opt.mode != TFSM_MODE_CAPPED && opt.mode != TFSM_MODE_EXACT)
+    if (opt.mode != TFSM_MODE_CAPPED && opt.mode != TFSM_MODE_EXACT && opt.mode != TFSM_MODE_FRACTION)
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown mode"};
+    const bool fraction = opt.mode == TFSM_MODE_FRACTION;
     if (opt.task_order != TFSM_ORDER_REFERENCE && opt.task_order != TFSM_ORDER_LEARNED)
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown task order"};
     if (opt.reserved != 0) throw ArgError{TFSM_ERR_ARGUMENT, "reserved option field must be 0"};
@@ -508,6 +545,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     hipStream_t s = c->stream;
     sync_spin(c);                       // nothing of an earlier call still reads the staging
     c->stage.reset();
+    if (fraction) ensure_scores(c);
     HIPX(hipEventRecord(c->ev[0], s));
 
     // ---------------- host description of every pattern ----------------
@@ -741,15 +779,33 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             task_base.push_back(task_base.back() + ntask);
         }
     }
+    // Fraction-Score sums: one item per 32 candidates of each vertex of a
+    // search- or express-decided pattern
+    HostWork frac_work;
+    if (fraction)
+        for (uint32_t p = 0; p < n; ++p)
+            if (decided[p] == TFSM_DECIDED_SEARCH || decided[p] == TFSM_DECIDED_EXPRESS)
+                for (uint32_t u = 0; u < hs[p].nv; ++u)
+                    frac_work.add(p, u | (decided[p] == TFSM_DECIDED_EXPRESS ? 1u << 8 : 0u),
+                                  (hs[p].dom_size[u] + 31) / 32);
+    const size_t frac_bytes = fraction ? (size_t)n * kMaxV * sizeof(FracSum) : 0;
     // this phase's uploads (the descriptors with their plans, the table
     // work, the task bases) in one copy, the verdict bitmaps' fill in one
     // launch; the later kernels take the new descriptor copy
     PhaseBatch b2(c, n * sizeof(PatStatic) + aux_work.bytes() + 8 * task_base.size() + 3 * 256 +
-                         sizeof(CopyDesc) * (verdict_words / kFillPiece + 32) + 512);
+                         (fraction ? frac_work.bytes() : 0) +
+                         sizeof(CopyDesc) * ((verdict_words + frac_bytes / 4) / kFillPiece + 34) + 512);
     d_ps = b2.add(hs.data(), n);
     const WorkTable waux = aux_work.to_device(b2);
     const unsigned long long *d_task_base = b2.add(task_base);
     if (verdict_words) b2.zero(verdict_region, verdict_words * sizeof(uint32_t));
+    FracSum *d_frac = nullptr;
+    WorkTable wfrac{};
+    if (fraction) {
+        d_frac = c->work.alloc<FracSum>((size_t)n * kMaxV);
+        b2.zero(d_frac, frac_bytes);
+        wfrac = frac_work.to_device(b2);
+    }
     // the learned order's per-vertex sums (and their probability-level
     // split) and the search counters start at zero
     // sums, then their probability-level split, then the sample pass's sums
@@ -964,8 +1020,18 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             hipLaunchKernelGGL(k_trace_final, dim3(gt), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks,
                                c->d_trace);
     }
+    if (fraction && wfrac.total) {
+        hipLaunchKernelGGL(k_frac_sum, dim3(grid_for(wfrac.total, c->num_cus)), dim3(kBlock), 0, s, d_ps, wfrac,
+                           (const double *)c->d_score, c->L, d_frac);
+        HIPX(hipGetLastError());
+    }
     HIPX(hipEventRecord(c->ev[3], s));
     HIPX(hipMemcpyAsync(hst, d_st, n * sizeof(PatState), hipMemcpyDeviceToHost, s));
+    FracSum *hfrac = nullptr;
+    if (fraction) {
+        hfrac = c->h_frac.ensure((size_t)n * kMaxV);
+        HIPX(hipMemcpyAsync(hfrac, d_frac, frac_bytes, hipMemcpyDeviceToHost, s));
+    }
     // the two launches' counters and the pool words follow each other: one copy
     SearchCounters *hctr = c->h_ctr.ensure(3);
     HIPX(hipMemcpyAsync(hctr, c->d_ctr, 2 * sizeof(SearchCounters) + 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1044,10 +1110,23 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         r.support_lower = lo;
         r.support_upper = hi;
         r.searches = S.searches;
-        if (decided[p] == TFSM_DECIDED_SEARCH && opt.mode == TFSM_MODE_CAPPED && S.status != 0)
+        if (fraction) {
+            // a pattern stopped by the valid-count bound (status 2) is
+            // infrequent; every other search- or express-decided one has all
+            // its verdicts, so its sums are exact
+            double mn = 0;
+            for (uint32_t u = 0; u < P.nv; ++u) {
+                r.fraction[u] = frac_value(hfrac[(size_t)p * kMaxV + u]);
+                mn = u == 0 ? r.fraction[u] : std::min(mn, r.fraction[u]);
+            }
+            r.fraction_support = mn;
+            r.frequent = decided[p] != TFSM_DECIDED_FILTER && !(decided[p] == TFSM_DECIDED_SEARCH && S.status == 2) &&
+                         mn >= (double)call.freq;
+        } else if (decided[p] == TFSM_DECIDED_SEARCH && opt.mode == TFSM_MODE_CAPPED && S.status != 0) {
             r.frequent = S.status == 1;
-        else
+        } else {
             r.frequent = decided[p] != TFSM_DECIDED_FILTER && lo >= call.freq;
+        }
         if (r.frequent && opt.retain_domains) {
             size_t words = 0;
             for (uint32_t u = 0; u < P.nv; ++u) words += P.dom_size[u] + (P.dom_size[u] + 31) / 32;
@@ -1240,6 +1319,7 @@ int tfsm_ctx_destroy(tfsm_ctx *c) {
     (void)hipFree(c->d_seg);
     (void)hipFree(c->d_nbr);
     (void)hipFree(c->d_bucket);
+    (void)hipFree(c->d_score);
     (void)hipFree(c->d_pool);
     (void)hipFree(c->d_ctr);
     (void)hipFree(c->d_records);
@@ -1317,6 +1397,23 @@ int tfsm_eval_level(tfsm_ctx *ctx, const tfsm_pattern *ps, const uint64_t *paren
         if (slices > 1) ctx->stats = sum;
         return TFSM_OK;
     });
+}
+
+int64_t tfsm_fraction_scores(tfsm_ctx *ctx, double *out, uint64_t cap) {
+    if (!ctx) return TFSM_ERR_ARGUMENT;
+    int64_t cells = 0;
+    const int rc = guarded(ctx, [&]() -> int {
+        HIPX(hipStreamSynchronize(ctx->stream));
+        ensure_scores(ctx);
+        cells = (int64_t)ctx->n * ctx->L;
+        const uint64_t k = std::min<uint64_t>(cap, (uint64_t)cells);
+        if (out && k) {
+            HIPX(hipMemcpyAsync(out, ctx->d_score, k * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            HIPX(hipStreamSynchronize(ctx->stream));
+        }
+        return TFSM_OK;
+    });
+    return rc != TFSM_OK ? rc : cells;
 }
 
 int tfsm_domain_read(tfsm_ctx *ctx, uint64_t handle, uint32_t u, uint32_t *out, uint32_t cap, uint32_t *size) {

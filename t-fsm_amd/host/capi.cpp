@@ -133,7 +133,12 @@ int64_t tfsmh_mine(const tfsmh_graph *h, tfsm_ctx *ctx, int freq, int max_nodes,
     if (keys_path) {
         FILE *f = std::fopen(keys_path, "w");
         if (f) {
-            for (auto &fp : found) std::fprintf(f, "%s\t%u\t%u\n", fp.p.key().c_str(), fp.support_lower, fp.support_upper);
+            // Fraction-Score runs add the support as a fourth column (hex float: every bit)
+            for (auto &fp : found) {
+                std::fprintf(f, "%s\t%u\t%u", fp.p.key().c_str(), fp.support_lower, fp.support_upper);
+                if (mode == TFSM_MODE_FRACTION) std::fprintf(f, "\t%a", fp.fraction);
+                std::fprintf(f, "\n");
+            }
             std::fclose(f);
         }
     }

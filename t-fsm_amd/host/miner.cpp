@@ -1,5 +1,6 @@
 // Level-synchronous miner; behaviour cited in miner.h.
 #include "miner.h"
+#include "fraction.h"
 #include "pushdown.h"
 #include "worker_pool.h"
 
@@ -75,14 +76,30 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
     if (opt.max_nodes < 0 || opt.max_nodes > TFSM_MAX_PATTERN_VERTICES)
         stats.capped_vertices = TFSM_MAX_PATTERN_VERTICES;
     for (int32_t l : g.elabel_values) gc.freq_edge_labels.insert(l);
-    // size-2 level (grami.h:83-169): every surviving endpoint-set entry is frequent
+    // size-2 level (grami.h:83-169): every surviving endpoint-set entry is
+    // frequent; with Fraction-Score support, those whose two summed scores
+    // reach freq (fraction-score/graph.h:803-824)
+    std::vector<std::pair<double, double>> frac2;
+    if (opt.mode == TFSM_MODE_FRACTION) {
+        const int64_t cells = tfsm_fraction_scores(ctx, nullptr, 0);
+        if (cells < 0) {
+            err = std::string("fraction scores: ") + tfsm_last_error(ctx);
+            return (int)cells;
+        }
+        std::vector<double> score((size_t)cells);
+        tfsm_fraction_scores(ctx, score.data(), (uint64_t)cells);
+        frac2 = edge_fraction(g, score);
+    }
     std::vector<Pattern> size2;
-    for (const EdgeStat &e : g.edge_stats) {
+    for (size_t i = 0; i < g.edge_stats.size(); ++i) {
+        const EdgeStat &e = g.edge_stats[i];
+        if (!frac2.empty() && (frac2[i].first < (double)opt.freq || frac2[i].second < (double)opt.freq)) continue;
         gc.neighbor_labels[e.la].insert(e.lb);
         gc.neighbor_labels[e.lb].insert(e.la);
         FoundPattern f;
         f.p = single_edge(e.la, e.el, e.lb);
         f.support_lower = f.support_upper = std::min(e.na, e.nb);
+        if (!frac2.empty()) f.fraction = std::min(frac2[i].first, frac2[i].second);
         size2.push_back(f.p);
         found.push_back(std::move(f));
     }
@@ -232,6 +249,7 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
             f.p = level[i].p;
             f.support_lower = res[i].support_lower;
             f.support_upper = res[i].support_upper;
+            f.fraction = res[i].fraction_support;
             found.push_back(std::move(f));
             for (auto &c : kids[i]) {
                 if (c.ne() > TFSM_MAX_PATTERN_EDGES) {

@@ -38,6 +38,7 @@ struct MinerOptions {
 struct FoundPattern {
     Pattern p;
     uint32_t support_lower = 0, support_upper = 0;
+    double fraction = 0;       // Fraction-Score support (TFSM_MODE_FRACTION)
 };
 
 struct LevelStats {

@@ -1,7 +1,7 @@
 // `run`: the miner's command line.
 //
 //   run -file G.lg -freq F [-thread T] [-maxNodes K]
-//       [-gpu plain|batched] [-support capped|exact] [-learned RANKER]
+//       [-gpu plain|batched] [-support capped|exact|fraction] [-learned RANKER]
 //       [-tune TUNING.json] [-out grami_results.txt] [-stats stats.json]
 //       [-csrCache 1] [-pushdown 1]
 //
@@ -12,7 +12,9 @@
 // the pruned graph from a binary cache beside the file (`<file>.f<freq>.csr`,
 // written on the first run and checked against the file's bytes).
 // -pushdown 1 turns on push-down pruning from cached frequent sub-patterns
-// (the reference's decompose.h / comper.h:1582-1612 cache).
+// (the reference's decompose.h / comper.h:1582-1612 cache).  -support
+// fraction mines with Fraction-Score support (the reference's fraction-score
+// build, same flags and output).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -42,7 +44,7 @@ int main(int argc, char **argv) {
     const char *freq_s = flag(argc, argv, "-freq");
     if (!file || !freq_s) {
         std::cerr << "usage: run -file G.lg -freq F [-thread T] [-maxNodes K] [-gpu plain|batched]"
-                     " [-support capped|exact] [-learned RANKER] [-tune TUNING] [-out PATH] [-stats PATH]\n";
+                     " [-support capped|exact|fraction] [-learned RANKER] [-tune TUNING] [-out PATH] [-stats PATH]\n";
         return 2;
     }
     tfsm::MinerOptions opt;
@@ -51,7 +53,12 @@ int main(int argc, char **argv) {
     if (const char *s = flag(argc, argv, "-thread")) opt.threads = std::max(1, std::atoi(s));
     if (const char *s = flag(argc, argv, "-maxNodes")) opt.max_nodes = std::atoi(s);
     if (const char *s = flag(argc, argv, "-gpu")) opt.batched = std::strcmp(s, "batched") == 0;
-    if (const char *s = flag(argc, argv, "-support")) opt.mode = std::strcmp(s, "exact") == 0 ? TFSM_MODE_EXACT : TFSM_MODE_CAPPED;
+    if (const char *s = flag(argc, argv, "-support")) {
+        if (std::strcmp(s, "capped") == 0) opt.mode = TFSM_MODE_CAPPED;
+        else if (std::strcmp(s, "exact") == 0) opt.mode = TFSM_MODE_EXACT;
+        else if (std::strcmp(s, "fraction") == 0) opt.mode = TFSM_MODE_FRACTION;
+        else { std::cerr << "run: -support must be capped, exact or fraction\n"; return 2; }
+    }
     if (const char *s = flag(argc, argv, "-pushdown")) opt.pushdown = std::strcmp(s, "0") != 0;
     const char *ranker_path = flag(argc, argv, "-learned");
     if (const char *t = flag(argc, argv, "-tune")) {

@@ -15,6 +15,7 @@ MAX_V = 16
 MAX_E = 64
 MODE_CAPPED = 0
 MODE_EXACT = 1
+MODE_FRACTION = 2
 ORDER_REFERENCE = 0
 ORDER_LEARNED = 1
 MATCH_ADAPTIVE = 0
@@ -59,7 +60,8 @@ class PatternResult(ctypes.Structure):
                 ("support_lower", ctypes.c_uint32), ("support_upper", ctypes.c_uint32),
                 ("domain_size", ctypes.c_uint32 * MAX_V), ("valid_count", ctypes.c_uint32 * MAX_V),
                 ("invalid_count", ctypes.c_uint32 * MAX_V), ("searches", ctypes.c_uint64),
-                ("verdicts", ctypes.c_uint64), ("domain_handle", ctypes.c_uint64)]
+                ("verdicts", ctypes.c_uint64), ("domain_handle", ctypes.c_uint64),
+                ("fraction", ctypes.c_double * MAX_V), ("fraction_support", ctypes.c_double)]
 
 
 class CallStats(ctypes.Structure):
@@ -98,7 +100,7 @@ GPU_SYMBOLS = ["tfsm_ctx_create", "tfsm_ctx_destroy", "tfsm_ctx_set_pool_capacit
                "tfsm_last_aux", "tfsm_last_order", "tfsm_read_trace", "tfsm_last_stats",
                "tfsm_status_string", "tfsm_last_error", "tfsm_ranker_set", "tfsm_ranker_get",
                "tfsm_ranker_forward", "tfsm_ranker_gradient", "tfsm_ranker_train", "tfsm_read_features",
-               "tfsm_ranker_gradient_noalloc"]
+               "tfsm_ranker_gradient_noalloc", "tfsm_fraction_scores"]
 
 _gpu = None
 _host = None
@@ -153,6 +155,8 @@ def gpu_lib():
         # device pointers (integers) for the no-allocation gradient
         lib.tfsm_ranker_gradient_noalloc.argtypes = [vp, vp, vp, ctypes.c_uint64, vp, vp]
         lib.tfsm_read_features.restype = ctypes.c_int64
+        lib.tfsm_fraction_scores.argtypes = [vp, P(ctypes.c_double), ctypes.c_uint64]
+        lib.tfsm_fraction_scores.restype = ctypes.c_int64
         _gpu = lib
     return _gpu
 
@@ -397,6 +401,15 @@ class Context:
         _check(self.h, self.lib.tfsm_ranker_train(self.h, X.ctypes.data_as(fp), y.ctypes.data_as(fp), len(X), epochs,
                                                   batch, lr, seed, ctypes.byref(loss)))
         return loss.value
+
+    def fraction_scores(self):
+        """The Fraction-Score table, (num_vertices, num_vlabels) float64."""
+        import numpy as np
+        n = self.lib.tfsm_fraction_scores(self.h, None, 0)
+        _check(self.h, 0 if n >= 0 else int(n))
+        out = np.zeros(max(n, 1), np.float64)
+        self.lib.tfsm_fraction_scores(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
+        return out[:n].reshape(self.graph.n, -1) if self.graph.n else out[:0].reshape(0, 0)
 
     def domain_read(self, handle, u):
         import numpy as np

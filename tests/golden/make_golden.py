@@ -1,6 +1,6 @@
 """Writes the golden pattern-set fixtures from the reference T-FSM binaries.
 
-Runs oracle/_ref/run_serial and oracle/_ref/run_parallel (built from the
+Runs oracle/_ref/run_serial, run_parallel and run_fraction (built from the
 reference sources by `make -f oracle/ref.mk`) on the seeded workloads written
 by t-fsm_amd/_build/tfsm_gen, and stores each frequent-pattern set as sorted
 DFS-code keys "from,to,from_label,edge_label,to_label;..." (one pattern per
@@ -13,7 +13,9 @@ The serial build writes grami_results.txt (serial/grami.h:622-635) with every
 frequent pattern.  The parallel build prints patterns of three or more
 vertices between `~~~~~` / `#####` markers (parallel/comper.h:1027-1031,
 1095-1101) and only counts the size-2 patterns, so its fixtures hold the
-patterns of three or more vertices plus the reported total.
+patterns of three or more vertices plus the reported total.  The
+Fraction-Score build (oracle/_ref/run_fraction, from fraction-score/) prints
+as the parallel build does.
 """
 import fcntl
 import hashlib
@@ -54,6 +56,12 @@ CONFIGS = {
     # is the graph the learned order is tested on (not trained on).
     "rmat20_f5300_m5": ("rmat20", 4, 5300, 5, "serial", 1),
     "rmat20s5_f5300_m5": ("rmat20", 5, 5300, 5, "serial", 1),
+    # Fraction-Score support (the reference's fraction-score build, which
+    # prints like the parallel build; one thread keeps its output lines whole)
+    "readme_frac_f1_m3": ("readme", 0, 1, 3, "fraction", 1),
+    "gnm2k_frac_f40_m4": ("gnm2k", 1, 40, 4, "fraction", 1),
+    "gnm2k_frac_f60_m4": ("gnm2k", 1, 60, 4, "fraction", 1),
+    "powerlaw100k_frac_f3000_m5": ("powerlaw100k", 2, 3000, 5, "fraction", 1),
 }
 
 
@@ -130,7 +138,8 @@ def main(names):
                                 os.path.join(HERE, name + ".grami_results.txt"))
                 reported = int([l for l in out.split("\n") if "# Frequent Patterns" in l][0].split(":")[1])
             else:
-                cmd = [os.path.join(REF, "run_parallel"), "-file", lg, "-freq", str(freq), "-thread", str(threads),
+                exe = "run_fraction" if build == "fraction" else "run_parallel"
+                cmd = [os.path.join(REF, exe), "-file", lg, "-freq", str(freq), "-thread", str(threads),
                        "-maxNodes", str(maxn)]
                 out = subprocess.run(cmd, cwd=tmp, check=True, capture_output=True, text=True).stdout
                 keys = parse_parallel(out)
