@@ -39,6 +39,10 @@ struct DevGraph {
     uint32_t EL;       // edge labels
     const uint32_t *seg;     // n * (K + 1)
     const uint32_t *nbr;     // partitioned adjacency
+    // per vertex, bit (k mod 64) set for every key k with a neighbour: a
+    // necessary condition for v to have a neighbour over key k, tested with
+    // one load before the segment offsets
+    const unsigned long long *keymask;
 };
 
 // The early exits a support mode allows the existence search: the infrequent
@@ -64,6 +68,7 @@ struct PatStatic {
     uint32_t nbr_n[kMaxV];
     uint8_t nbr_v[kMaxV][kMaxV];        // pattern neighbours of u
     uint32_t nbr_key[kMaxV][kMaxV];     // segment key of the edge u - nbr_v
+    unsigned long long need_mask[kMaxV];   // keymask bits every candidate of u must have
     uint32_t nlf_n[kMaxV];
     uint32_t nlf_lab[kMaxV][kMaxV];
     uint32_t nlf_cnt[kMaxV][kMaxV];

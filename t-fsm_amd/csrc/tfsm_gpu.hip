@@ -217,6 +217,7 @@ struct tfsm_ctx {
     uint32_t n = 0, L = 0, EL = 0, K = 0, W = 0;
     uint64_t nadj = 0;
     uint32_t *d_seg = nullptr, *d_nbr = nullptr, *d_bucket = nullptr;
+    unsigned long long *d_keymask = nullptr; // per vertex: keys (mod 64) with a neighbour
     double *d_score = nullptr;               // Fraction-Score table, n x L (computed on first use)
     std::vector<uint32_t> bucket_off;        // host copy, L + 1
     DevGraph g{};
@@ -581,6 +582,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                 if (!adj[a][b]) continue;
                 P.nbr_v[a][P.nbr_n[a]] = (uint8_t)b;
                 P.nbr_key[a][P.nbr_n[a]] = P.vlab[b] * c->EL + el_of[a][b];
+                P.need_mask[a] |= 1ull << (P.nbr_key[a][P.nbr_n[a]] & 63);
                 P.nbr_n[a]++;
                 uint32_t k = 0;
                 while (k < nl && lab_of[k] != P.vlab[b]) ++k;
@@ -1228,6 +1230,7 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         std::vector<uint32_t> seg((size_t)c->n * (c->K + 1));
         std::vector<uint32_t> nbr(g->num_adj);
         std::vector<uint32_t> cnt(c->K + 1);
+        std::vector<unsigned long long> keymask(c->n, 0);
         for (uint32_t v = 0; v < c->n; ++v) {
             const uint32_t b = g->offsets[v], e = g->offsets[v + 1];
             std::fill(cnt.begin(), cnt.end(), 0);
@@ -1242,6 +1245,7 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
             for (uint32_t k = 0; k < c->K; ++k) {
                 sv[k] = acc;
                 acc += cnt[k + 1];
+                if (cnt[k + 1]) keymask[v] |= 1ull << (k & 63);
             }
             sv[c->K] = acc;
             std::vector<uint32_t> fill(sv, sv + c->K);
@@ -1261,6 +1265,10 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         HIPX(hipMalloc(&c->d_seg, seg.size() * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_nbr, std::max<size_t>(nbr.size(), 1) * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_bucket, bucket.size() * sizeof(uint32_t)));
+        HIPX(hipMalloc(&c->d_keymask, std::max<size_t>(keymask.size(), 1) * sizeof(unsigned long long)));
+        if (!keymask.empty())
+            HIPX(hipMemcpy(c->d_keymask, keymask.data(), keymask.size() * sizeof(unsigned long long),
+                           hipMemcpyHostToDevice));
         HIPX(hipMemcpy(c->d_seg, seg.data(), seg.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         if (!nbr.empty()) HIPX(hipMemcpy(c->d_nbr, nbr.data(), nbr.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         HIPX(hipMemcpy(c->d_bucket, bucket.data(), bucket.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -1270,6 +1278,7 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         c->g.EL = c->EL;
         c->g.seg = c->d_seg;
         c->g.nbr = c->d_nbr;
+        c->g.keymask = c->d_keymask;
         c->pool_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * g->num_adj, 1u << 22), 0xfffffff0ull);
         HIPX(hipMalloc(&c->d_pool, (size_t)c->pool_cap * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_pflags, kMaxPrunePasses * sizeof(uint32_t)));
@@ -1319,6 +1328,7 @@ int tfsm_ctx_destroy(tfsm_ctx *c) {
     (void)hipFree(c->d_seg);
     (void)hipFree(c->d_nbr);
     (void)hipFree(c->d_bucket);
+    (void)hipFree(c->d_keymask);
     (void)hipFree(c->d_score);
     (void)hipFree(c->d_pool);
     (void)hipFree(c->d_ctr);

@@ -161,9 +161,16 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
         // level while the GPU evaluates it (growth is pure: a pattern's
         // children depend on the pattern alone); only the frequent ones'
         // children are kept, so the candidates are those of growing after
-        // the verdicts
+        // the verdicts.  Once the verdicts are in, the patterns not yet grown
+        // that are infrequent are skipped (on a level of 10^5 patterns of
+        // which 1% are frequent, growing them all costs more than the GPU)
         std::vector<std::vector<Pattern>> kids(n);
-        Background grower(n, opt.threads, [&](size_t i) { kids[i] = extend(level[i].p, gc); });
+        std::unique_ptr<std::atomic<uint8_t>[]> skip(new std::atomic<uint8_t>[n]);
+        for (size_t i = 0; i < n; ++i) skip[i].store(0, std::memory_order_relaxed);
+        Background grower(n, opt.threads, [&](size_t i) {
+            if (!skip[i].load(std::memory_order_relaxed)) kids[i] = extend(level[i].p, gc);
+        });
+        const auto te = std::chrono::steady_clock::now();
         auto account = [&]() {
             tfsm_call_stats cs;
             tfsm_last_stats(ctx, &cs);
@@ -216,6 +223,9 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
                 account();
             }
         }
+        stats.eval_wall_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - te).count();
+        for (size_t i = 0; i < n; ++i)
+            if (!res[i].frequent) skip[i].store(1, std::memory_order_relaxed);
         // the parents' domains are no longer needed once all their children ran
         // (with push-down pruning they stay cached until the run ends)
         if (!opt.pushdown) {
@@ -241,7 +251,9 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
             freq_idx.push_back(i);
         }
         ls.frequent = freq_idx.size();
+        const auto tw = std::chrono::steady_clock::now();
         grower.wait();
+        stats.growth_wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
         std::vector<Candidate> next;
         for (size_t k = 0; k < freq_idx.size(); ++k) {
             const size_t i = freq_idx[k];

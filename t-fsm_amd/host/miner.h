@@ -63,6 +63,8 @@ struct MinerStats {
     uint64_t dropped_large = 0;     // children over the device's edge limit, not evaluated
     uint64_t pushdown_sources = 0;  // push-down entries handed to the evaluation
     uint64_t domain_sum = 0;        // filtered domain sizes summed over the evaluated patterns
+    double eval_wall_ms = 0;        // host wall time inside the eval calls
+    double growth_wait_ms = 0;      // host wall time waiting for the next level's growth
 };
 
 // Runs the whole mining job; `found` receives the frequent patterns in a

@@ -127,12 +127,13 @@ int main(int argc, char **argv) {
             std::fprintf(f,
                          "{\"mining_s\": %.6f, \"evaluated\": %llu, \"frequent\": %llu, \"verdicts\": %llu, "
                          "\"searches\": %llu, \"eval_calls\": %llu, \"filter_ms\": %.4f, \"table_ms\": %.4f, "
-                         "\"search_ms\": %.4f, \"search_launches\": %llu, \"bytes_table\": %llu, \"bytes_search\": %llu}\n",
+                         "\"search_ms\": %.4f, \"search_launches\": %llu, \"bytes_table\": %llu, \"bytes_search\": %llu, "
+                         "\"eval_wall_ms\": %.3f, \"growth_wait_ms\": %.3f}\n",
                          st.mining_s, (unsigned long long)st.evaluated, (unsigned long long)st.frequent,
                          (unsigned long long)st.verdicts, (unsigned long long)st.searches,
                          (unsigned long long)st.eval_calls, st.gpu_ms[0], st.gpu_ms[1], st.gpu_ms[2],
                          (unsigned long long)st.search_launches, (unsigned long long)st.bytes_table,
-                         (unsigned long long)st.bytes_search);
+                         (unsigned long long)st.bytes_search, st.eval_wall_ms, st.growth_wait_ms);
             std::fclose(f);
         }
     }

@@ -10,6 +10,14 @@ and the learned eval order.
 
     python t-fsm_amd/train_ranker.py --out t-fsm_amd/rankers/powerlaw.f32 \\
         --workload powerlaw100k --seeds 11,12 --freqs 9000,7000 --max-nodes 5
+
+Levels of tens of thousands of patterns over a million-vertex graph (R-MAT
+scale 20) have billions of tasks; `--sample-patterns N` traces a seeded
+sample of at most N patterns of each level in exact mode and grows the
+next level from the whole level evaluated in capped mode:
+
+    python t-fsm_amd/train_ranker.py --out t-fsm_amd/rankers/rmat.f32 \\
+        --workload rmat20 --seeds 11,12 --freqs 5300 --max-nodes 5 --sample-patterns 150
 """
 import argparse
 import json
@@ -35,8 +43,11 @@ def dense_pattern(codes, vv, ev):
                              [(f, t, ev.index(el)) for f, t, fl, el, tl in codes])
 
 
-def level_traces(graph, ctx, freq, max_nodes, mode=tfsm.MODE_EXACT):
-    """Mines level by level (batched) and yields (features, final states, steps) per level."""
+def level_traces(graph, ctx, freq, max_nodes, mode=tfsm.MODE_EXACT, sample=0, seed=0):
+    """Mines level by level (batched) and yields (features, final states, steps) per level.
+    sample > 0: trace at most that many patterns of each level (a seeded
+    choice) and grow from the whole level evaluated in capped mode."""
+    rng = np.random.default_rng(seed)
     csr = graph.csr()
     vv = [int(x) for x in csr["vlabel_values"]]
     ev = [int(x) for x in csr["elabel_values"]]
@@ -51,10 +62,19 @@ def level_traces(graph, ctx, freq, max_nodes, mode=tfsm.MODE_EXACT):
                 keep.append((k, h))
         if not pats:
             break
-        res = ctx.eval_level(pats, freq, mode=mode, parents=[h for _, h in keep], retain=True, trace=True)
-        tr = ctx.read_trace()
-        X = ctx.read_features()
-        yield X, tr >> 30, tr & ((1 << 30) - 1)
+        parents = [h for _, h in keep]
+        if sample and len(pats) > sample:
+            pick = np.sort(rng.choice(len(pats), sample, replace=False))
+            ctx.eval_level([pats[i] for i in pick], freq, mode=mode, parents=[parents[i] for i in pick], trace=True)
+            tr = ctx.read_trace()
+            X = ctx.read_features()
+            yield X, tr >> 30, tr & ((1 << 30) - 1)
+            res = ctx.eval_level(pats, freq, mode=tfsm.MODE_CAPPED, parents=parents, retain=True)
+        else:
+            res = ctx.eval_level(pats, freq, mode=mode, parents=parents, retain=True, trace=True)
+            tr = ctx.read_trace()
+            X = ctx.read_features()
+            yield X, tr >> 30, tr & ((1 << 30) - 1)
         for h in handles:
             ctx.domain_release(h)
         handles = [r.domain_handle for r in res if r.frequent and r.domain_handle]
@@ -67,7 +87,7 @@ def level_traces(graph, ctx, freq, max_nodes, mode=tfsm.MODE_EXACT):
         ctx.domain_release(h)
 
 
-def collect(workload, seeds, freqs, max_nodes, tmp, per_level=400_000):
+def collect(workload, seeds, freqs, max_nodes, tmp, per_level=400_000, sample=0):
     """Decided tasks' (features, valid) rows; at most per_level rows sampled per level."""
     rng = np.random.default_rng(0)
     Xs, ys = [], []
@@ -78,7 +98,7 @@ def collect(workload, seeds, freqs, max_nodes, tmp, per_level=400_000):
         for freq in freqs:
             g = tfsm.Graph(lg, freq)
             ctx = tfsm.Context(g)
-            for X, state, _ in level_traces(g, ctx, freq, max_nodes):
+            for X, state, _ in level_traces(g, ctx, freq, max_nodes, sample=sample, seed=seed):
                 keep = np.flatnonzero(state > 0)
                 if len(keep) > per_level:
                     keep = np.sort(rng.choice(keep, per_level, replace=False))
@@ -133,11 +153,13 @@ def main():
     ap.add_argument("--freqs", default="9000,7000")
     ap.add_argument("--max-nodes", type=int, default=5)
     ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--sample-patterns", type=int, default=0,
+                    help="trace at most this many patterns per level (0: every pattern)")
     args = ap.parse_args()
     seeds = [int(s) for s in args.seeds.split(",")]
     freqs = [int(f) for f in args.freqs.split(",")]
     with tempfile.TemporaryDirectory() as tmp:
-        X, y = collect(args.workload, seeds, freqs, args.max_nodes, tmp)
+        X, y = collect(args.workload, seeds, freqs, args.max_nodes, tmp, sample=args.sample_patterns)
         lg = os.path.join(tmp, f"{args.workload}_{seeds[0]}.lg")
         g = tfsm.Graph(lg, freqs[0])
         ctx = tfsm.Context(g)
@@ -145,7 +167,8 @@ def main():
         ctx.close()
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     w.astype("<f4").tofile(args.out)
-    info.update(workload=args.workload, seeds=seeds, freqs=freqs, max_nodes=args.max_nodes, epochs=args.epochs)
+    info.update(workload=args.workload, seeds=seeds, freqs=freqs, max_nodes=args.max_nodes, epochs=args.epochs,
+                sample_patterns=args.sample_patterns)
     with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
         json.dump(info, f, indent=1)
     print(json.dumps(info))
