@@ -75,6 +75,9 @@ struct PatStatic {
     // LDF sources: parent domain (with its invalid bits) or label bucket
     const uint32_t *src[kMaxV];
     const uint32_t *src_invalid[kMaxV];
+    // label bucket sources: the key masks of the bucket's vertices in bucket
+    // order (a coalesced read beside src); null for parent domains
+    const unsigned long long *src_mask[kMaxV];
     uint32_t src_n[kMaxV];
     // domains
     // per 32 data vertices, a pair of words: the membership bits of C(u),

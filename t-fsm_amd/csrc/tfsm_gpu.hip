@@ -218,6 +218,7 @@ struct tfsm_ctx {
     uint64_t nadj = 0;
     uint32_t *d_seg = nullptr, *d_nbr = nullptr, *d_bucket = nullptr;
     unsigned long long *d_keymask = nullptr; // per vertex: keys (mod 64) with a neighbour
+    unsigned long long *d_keymask_bucket = nullptr;  // the same in label-bucket order
     double *d_score = nullptr;               // Fraction-Score table, n x L (computed on first use)
     std::vector<uint32_t> bucket_off;        // host copy, L + 1
     DevGraph g{};
@@ -602,6 +603,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                 P.src_n[u] = par->size[u];
             } else {
                 P.src[u] = c->d_bucket + c->bucket_off[P.vlab[u]];
+                P.src_mask[u] = c->d_keymask_bucket + c->bucket_off[P.vlab[u]];
                 P.src_invalid[u] = nullptr;
                 P.src_n[u] = c->bucket_off[P.vlab[u] + 1] - c->bucket_off[P.vlab[u]];
             }
@@ -1266,9 +1268,15 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         HIPX(hipMalloc(&c->d_nbr, std::max<size_t>(nbr.size(), 1) * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_bucket, bucket.size() * sizeof(uint32_t)));
         HIPX(hipMalloc(&c->d_keymask, std::max<size_t>(keymask.size(), 1) * sizeof(unsigned long long)));
-        if (!keymask.empty())
+        HIPX(hipMalloc(&c->d_keymask_bucket, std::max<size_t>(keymask.size(), 1) * sizeof(unsigned long long)));
+        if (!keymask.empty()) {
             HIPX(hipMemcpy(c->d_keymask, keymask.data(), keymask.size() * sizeof(unsigned long long),
                            hipMemcpyHostToDevice));
+            std::vector<unsigned long long> kb(c->n);
+            for (uint32_t i = 0; i < c->n; ++i) kb[i] = keymask[bucket[i]];
+            HIPX(hipMemcpy(c->d_keymask_bucket, kb.data(), kb.size() * sizeof(unsigned long long),
+                           hipMemcpyHostToDevice));
+        }
         HIPX(hipMemcpy(c->d_seg, seg.data(), seg.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         if (!nbr.empty()) HIPX(hipMemcpy(c->d_nbr, nbr.data(), nbr.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         HIPX(hipMemcpy(c->d_bucket, bucket.data(), bucket.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -1329,6 +1337,7 @@ int tfsm_ctx_destroy(tfsm_ctx *c) {
     (void)hipFree(c->d_nbr);
     (void)hipFree(c->d_bucket);
     (void)hipFree(c->d_keymask);
+    (void)hipFree(c->d_keymask_bucket);
     (void)hipFree(c->d_score);
     (void)hipFree(c->d_pool);
     (void)hipFree(c->d_ctr);

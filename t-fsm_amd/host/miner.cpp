@@ -33,6 +33,9 @@ tfsm_csr csr_view(const DataGraph &g) {
 
 namespace {
 
+// largest level whose children are grown while the GPU evaluates it
+constexpr size_t kSpeculativeGrowth = 4096;
+
 struct Candidate {
     Pattern p;
     uint64_t parent_handle = 0;
@@ -113,6 +116,8 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
         for (auto &k : kids)
             for (auto &c : k) level.push_back(Candidate{std::move(c), 0});
     }
+    // the size-2 level's wall time includes growing its children
+    stats.levels[0].wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     tfsm_eval_options eo = opt.tuning;
     eo.mode = (uint32_t)opt.mode;
     eo.retain_domains = 1;
@@ -162,15 +167,19 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
         // children depend on the pattern alone); only the frequent ones'
         // children are kept, so the candidates are those of growing after
         // the verdicts.  Once the verdicts are in, the patterns not yet grown
-        // that are infrequent are skipped (on a level of 10^5 patterns of
-        // which 1% are frequent, growing them all costs more than the GPU)
+        // that are infrequent are skipped.  Levels of more than
+        // kSpeculativeGrowth patterns are grown after the verdicts only: on a
+        // level of 10^5 patterns of which 1% are frequent, growing them all
+        // makes millions of children and costs more than the GPU
         std::vector<std::vector<Pattern>> kids(n);
         std::unique_ptr<std::atomic<uint8_t>[]> skip(new std::atomic<uint8_t>[n]);
-        for (size_t i = 0; i < n; ++i) skip[i].store(0, std::memory_order_relaxed);
-        Background grower(n, opt.threads, [&](size_t i) {
+        const bool speculate = n <= kSpeculativeGrowth;
+        for (size_t i = 0; i < n; ++i) skip[i].store(speculate ? 0 : 1, std::memory_order_relaxed);
+        Background grower(speculate ? n : 0, opt.threads, [&](size_t i) {
             if (!skip[i].load(std::memory_order_relaxed)) kids[i] = extend(level[i].p, gc);
         });
         const auto te = std::chrono::steady_clock::now();
+        const double prep_ms = std::chrono::duration<double, std::milli>(te - tl).count();
         auto account = [&]() {
             tfsm_call_stats cs;
             tfsm_last_stats(ctx, &cs);
@@ -223,7 +232,9 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
                 account();
             }
         }
-        stats.eval_wall_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - te).count();
+        const auto tp = std::chrono::steady_clock::now();
+        const double eval_ms = std::chrono::duration<double, std::milli>(tp - te).count();
+        stats.eval_wall_ms += eval_ms;
         for (size_t i = 0; i < n; ++i)
             if (!res[i].frequent) skip[i].store(1, std::memory_order_relaxed);
         // the parents' domains are no longer needed once all their children ran
@@ -253,6 +264,8 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
         ls.frequent = freq_idx.size();
         const auto tw = std::chrono::steady_clock::now();
         grower.wait();
+        if (!speculate)
+            parallel_for(freq_idx.size(), opt.threads, [&](size_t k) { kids[freq_idx[k]] = extend(level[freq_idx[k]].p, gc); });
         stats.growth_wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
         std::vector<Candidate> next;
         for (size_t k = 0; k < freq_idx.size(); ++k) {
@@ -272,7 +285,11 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
             }
         }
         stats.frequent += freq_idx.size();
-        ls.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tl).count();
+        const auto tn = std::chrono::steady_clock::now();
+        ls.wall_ms = std::chrono::duration<double, std::milli>(tn - tl).count();
+        ls.prep_ms = prep_ms;
+        ls.eval_ms = eval_ms;
+        ls.post_ms = std::chrono::duration<double, std::milli>(tn - tp).count();
         stats.levels.push_back(ls);
         level.swap(next);
     }

@@ -45,6 +45,9 @@ struct LevelStats {
     uint32_t size = 0;          // pattern vertex count of the level
     uint64_t evaluated = 0, frequent = 0;
     double wall_ms = 0;
+    double prep_ms = 0;         // host: device patterns and growth start, before the eval calls
+    double eval_ms = 0;         // host wall time inside the level's eval calls
+    double post_ms = 0;         // host: verdicts, retained handles and the next level's candidates
 };
 
 struct MinerStats {

@@ -128,12 +128,21 @@ int main(int argc, char **argv) {
                          "{\"mining_s\": %.6f, \"evaluated\": %llu, \"frequent\": %llu, \"verdicts\": %llu, "
                          "\"searches\": %llu, \"eval_calls\": %llu, \"filter_ms\": %.4f, \"table_ms\": %.4f, "
                          "\"search_ms\": %.4f, \"search_launches\": %llu, \"bytes_table\": %llu, \"bytes_search\": %llu, "
-                         "\"eval_wall_ms\": %.3f, \"growth_wait_ms\": %.3f}\n",
+                         "\"eval_wall_ms\": %.3f, \"growth_wait_ms\": %.3f, \"levels\": [",
                          st.mining_s, (unsigned long long)st.evaluated, (unsigned long long)st.frequent,
                          (unsigned long long)st.verdicts, (unsigned long long)st.searches,
                          (unsigned long long)st.eval_calls, st.gpu_ms[0], st.gpu_ms[1], st.gpu_ms[2],
                          (unsigned long long)st.search_launches, (unsigned long long)st.bytes_table,
                          (unsigned long long)st.bytes_search, st.eval_wall_ms, st.growth_wait_ms);
+            for (size_t i = 0; i < st.levels.size(); ++i) {
+                const tfsm::LevelStats &l = st.levels[i];
+                std::fprintf(f,
+                             "%s{\"vertices\": %u, \"evaluated\": %llu, \"frequent\": %llu, \"wall_ms\": %.3f, "
+                             "\"prep_ms\": %.3f, \"eval_ms\": %.3f, \"post_ms\": %.3f}",
+                             i ? ", " : "", l.size, (unsigned long long)l.evaluated, (unsigned long long)l.frequent,
+                             l.wall_ms, l.prep_ms, l.eval_ms, l.post_ms);
+            }
+            std::fprintf(f, "]}\n");
             std::fclose(f);
         }
     }
