@@ -70,6 +70,20 @@ def host_threads():
     return host_info()["threads"]
 
 
+def profiled_traffic(workload):
+    """HBM bytes per search launch from the committed counter summary of this
+    workload (two rocprofv3 --pmc runs, FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0,
+    the scales a 256 MiB copy gives), or None when there is none."""
+    path = os.path.join(REPO, "profiles", f"k_verify_{workload}.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch = "):
+                return int(line.split("=")[1]), os.path.relpath(path, REPO)
+    return None, None
+
+
 def golden_keys(workload):
     path = os.path.join(REPO, "tests", "golden", workload + ".keys")
     if not os.path.exists(path):
@@ -216,6 +230,8 @@ def main():
     bytes_per_launch = agg["bytes_search"] / max(1, agg["search_launches"])
     achieved = bytes_per_launch / (ms_search * 1e-3) / 1e9 if ms_search > 0 else 0.0
 
+    traffic, traffic_src = profiled_traffic(args.workload)
+
     cpu = None
     if not args.no_cpu_baseline:
         host = host_info()
@@ -275,7 +291,10 @@ def main():
         "speedup_batched_vs_plain": round(batch_value / plain_value, 3) if plain_value and batch_value else None,
         "roofline": {"bound": "hbm", "kernel": "k_verify (existence search)", "achieved": round(achieved, 3),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 6),
-                     "traffic": None, "launches": agg["search_launches"] // K,
+                     "traffic": traffic,
+                     "traffic_source": (f"{traffic_src}: rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), "
+                                        "mean per k_verify dispatch of the same path") if traffic else None,
+                     "launches": agg["search_launches"] // K,
                      "avg_launch_ms": round(ms_search, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)},
         "cpu_baseline": cpu,
     }
