@@ -139,6 +139,11 @@ typedef struct {
     uint32_t reserved;          /* must be 0 */
     uint32_t num_pushdown;      /* entries of pushdown (0: none) */
     const tfsm_pushdown *pushdown;
+    /* more of the search kernel's shape; none changes a verdict */
+    uint32_t steps_per_round;   /* DFS steps a lane takes between bookkeeping rounds (0: 8; at most 64) */
+    uint32_t refill_min;        /* fewest and most lanes' worth of root tasks a wave takes from the */
+    uint32_t refill_max;        /*   launch's counter at once (0: 128 and 256; 1 <= min <= max <= 4096) */
+    uint32_t prune_batch;       /* DP-iso passes launched between convergence checks (0: 6; at most 64) */
 } tfsm_eval_options;
 
 /* decided_by */

@@ -1,18 +1,22 @@
 """Autotuner for the search kernel's shape on one workload.
 
-Times whole mining runs (graph resident in HBM) for each candidate shape
-(resident waves per CU, step budget of the record splitting) and keeps the
-fastest.  A shape is accepted only if its run returns exactly the pattern set
-of the first (default) shape: the search's verdicts never depend on its shape,
-and the check makes sure.  Writes a tuning file that `run -tune FILE`,
-`Context.mine(..., **tfsm.load_tuning(FILE))` and bench.py apply.
+Times whole mining runs (graph resident in HBM) for candidate shapes and keeps
+the fastest.  The knobs are the resident waves per CU, the step budget of the
+record splitting, the lanes of a lane group, the DFS steps between bookkeeping
+rounds, the sizes of a wave's root-task refills and the DP-iso passes
+launched between convergence checks.  The space is searched one knob at a
+time from the library's defaults (coordinate descent, two sweeps), keeping
+each knob's best value before the next.  A shape is accepted only if its run
+returns exactly the pattern set of the first (default) shape: the search's
+verdicts never depend on its shape, and the check makes sure.  Writes a tuning
+file that `run -tune FILE`, `Context.mine(..., **tfsm.load_tuning(FILE))` and
+bench.py apply.
 
     python t-fsm_amd/autotune.py --workload powerlaw100k --seed 2 --freq 9000 \\
         --max-nodes 5 --ranker t-fsm_amd/rankers/powerlaw.f32 \\
         --out t-fsm_amd/tuning/powerlaw100k_f9000_m5.json
 """
 import argparse
-import itertools
 import json
 import os
 import statistics
@@ -25,12 +29,20 @@ sys.path.insert(0, HERE)
 import tfsm  # noqa: E402
 
 NEVER = 0xffffffff
-# Shapes the tuner may choose from: workgroups of 4 waves, 1 or 2 resident per
-# CU (the library clamps to what stays resident), and record splitting after
-# 64 to 1024 steps with one record per donation, or none.
+# Values the tuner may choose from, the library's default first: workgroups of
+# 4 waves, 1 or 2 resident per CU (the library clamps to what stays
+# resident); record splitting after 64 to 1024 steps, or none; lane groups of
+# up to 1-16 lanes; 2-16 steps per bookkeeping round; refills of 16-128 (at
+# least) and 128-1024 (at most) lanes' worth of tasks; 2-9 filter passes per
+# convergence check.
 SPACE = {
     "grid_waves_per_cu": [4, 8],
-    "split_steps": [64, 128, 256, 1024, NEVER],
+    "split_steps": [128, 64, 256, 1024, NEVER],
+    "group_lanes": [8, 1, 2, 4, 16],
+    "steps_per_round": [8, 2, 4, 16],
+    "refill_min": [128, 16, 32, 64],
+    "refill_max": [256, 128, 512, 1024],
+    "prune_batch": [6, 2, 4, 9],
 }
 
 
@@ -63,9 +75,12 @@ def main():
     if learned:
         ctx.ranker_load(args.ranker)
 
-    shapes = [dict(zip(SPACE, v)) for v in itertools.product(*SPACE.values())]
-    reference_keys, results = None, []
-    for shape in shapes:
+    reference_keys, results, seen = None, [], {}
+
+    def measure(shape):
+        key = tuple(sorted(shape.items()))
+        if key in seen:
+            return seen[key]
         kp = os.path.join(tmp, "keys.txt")
         times = []
         ctx.mine(args.freq, args.max_nodes, threads=16, batched=True, learned=learned, **shape)   # warm-up
@@ -74,14 +89,26 @@ def main():
                              keys_path=kp if r == 0 else None, **shape)
             times.append(st["mining_s"])
         keys = keys_of(kp)
+        nonlocal reference_keys
         if reference_keys is None:
             reference_keys = keys
-        ok = keys == reference_keys
-        results.append(dict(options=shape, median_ms=round(1e3 * statistics.median(times), 3),
-                            min_ms=round(1e3 * min(times), 3), same_patterns=ok))
-        print(json.dumps(results[-1]), flush=True)
-    good = [r for r in results if r["same_patterns"]]
-    best = min(good, key=lambda r: r["median_ms"])
+        res = dict(options=dict(shape), median_ms=round(1e3 * statistics.median(times), 3),
+                   min_ms=round(1e3 * min(times), 3), same_patterns=keys == reference_keys)
+        results.append(res)
+        seen[key] = res
+        print(json.dumps(res), flush=True)
+        return res
+
+    best = measure({k: v[0] for k, v in SPACE.items()})
+    for sweep in range(2):
+        for knob, values in SPACE.items():
+            for v in values:
+                shape = dict(best["options"], **{knob: v})
+                if shape["refill_min"] > shape["refill_max"]:
+                    continue
+                r = measure(shape)
+                if r["same_patterns"] and r["median_ms"] < best["median_ms"]:
+                    best = r
     out = dict(workload=dict(name=args.lg or args.workload, seed=args.seed, freq=args.freq,
                              max_nodes=args.max_nodes, learned=learned),
                options=best["options"], measured=results)

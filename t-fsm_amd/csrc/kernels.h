@@ -203,6 +203,7 @@ struct VerifyArgs {
     uint32_t part;                         // 0: every entry, 1: the first part, 2: the rest
     uint32_t abort_stopped;                // capped mode: abandon tasks of a u that has freq valid
     uint32_t steps_per_round;
+    uint32_t refill_min, refill_max;     // lanes' worth of root tasks per wave refill
     uint32_t split_steps;                  // donate work after this many steps (0: never)
     uint32_t split_fanout;                 // records per donation (>= 1)
     SplitRecord *records;                  // split_cap records

@@ -204,6 +204,12 @@ struct PinnedStage {
 
 constexpr uint32_t kMaxPrunePasses = 4096;   // filter passes of one call
 constexpr uint32_t kPruneBatch = 6;          // passes launched between convergence checks
+// search steps between bookkeeping rounds, and a wave's root-task refills in
+// lanes' worth (the autotuner's choices on both power-law workloads: 8 or 16
+// steps and refills of at least 128 were 2-10% faster than 4 and 64)
+constexpr uint32_t kDefaultStepsPerRound = 8;
+constexpr uint32_t kDefaultRefillMin = 128;
+constexpr uint32_t kDefaultRefillMax = 256;
 constexpr uint32_t kPrioBlocks = 512;        // most blocks of the priority sort (one histogram column each)
 
 }  // namespace
@@ -529,6 +535,14 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         throw ArgError{TFSM_ERR_ARGUMENT, "group_lanes must be 0 or a power of two up to 64"};
     if (opt.match_order != TFSM_MATCH_ADAPTIVE && opt.match_order != TFSM_MATCH_REFERENCE)
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown matching order"};
+    if (opt.steps_per_round > 64) throw ArgError{TFSM_ERR_ARGUMENT, "steps_per_round must be at most 64"};
+    {
+        const uint32_t lo = opt.refill_min ? opt.refill_min : kDefaultRefillMin;
+        const uint32_t hi = opt.refill_max ? opt.refill_max : kDefaultRefillMax;
+        if (lo > hi || hi > 4096) throw ArgError{TFSM_ERR_ARGUMENT, "refill_min <= refill_max <= 4096 required"};
+    }
+    if (opt.prune_batch > 64) throw ArgError{TFSM_ERR_ARGUMENT, "prune_batch must be at most 64"};
+    const uint32_t prune_batch = opt.prune_batch ? opt.prune_batch : kPruneBatch;
     const bool learned = opt.task_order == TFSM_ORDER_LEARNED;
     if (learned && !c->ranker_installed)
         throw ArgError{TFSM_ERR_ARGUMENT, "learned task order requires an installed ranker"};
@@ -715,7 +729,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     const uint32_t *h_more = reinterpret_cast<const uint32_t *>(hst + n);
     uint32_t passes = 0;
     for (;;) {
-        for (uint32_t i = 0; wsrc.total && i < kPruneBatch; ++i, ++passes) {
+        for (uint32_t i = 0; wsrc.total && i < prune_batch; ++i, ++passes) {
             if (passes >= kMaxPrunePasses) throw ArgError{TFSM_ERR_LIMIT, "filter did not converge"};
             hipLaunchKernelGGL(k_prune, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc, c->d_pflags + passes,
                                passes ? c->d_pflags + passes - 1 : nullptr,
@@ -866,7 +880,9 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         va.aux_region = aux_region;
         va.overflow = c->d_flags + 1;
         va.trace = opt.trace ? c->d_trace : nullptr;
-        va.steps_per_round = 4;
+        va.steps_per_round = opt.steps_per_round ? opt.steps_per_round : kDefaultStepsPerRound;
+        va.refill_min = opt.refill_min ? opt.refill_min : kDefaultRefillMin;
+        va.refill_max = opt.refill_max ? opt.refill_max : kDefaultRefillMax;
         va.split_steps = opt.split_steps == 0 ? kDefaultSplitSteps : opt.split_steps;
         if (opt.split_steps == 0xffffffffu) va.split_steps = 0;
         va.split_fanout = opt.split_fanout == 0 ? kDefaultSplitFanout : std::min(opt.split_fanout, 64u);

@@ -21,6 +21,10 @@ uint32_t *field(tfsm_eval_options &o, const std::string &name) {
     if (name == "long_fanout") return &o.long_fanout;
     if (name == "group_lanes") return &o.group_lanes;
     if (name == "match_order") return &o.match_order;
+    if (name == "steps_per_round") return &o.steps_per_round;
+    if (name == "refill_min") return &o.refill_min;
+    if (name == "refill_max") return &o.refill_max;
+    if (name == "prune_batch") return &o.prune_batch;
     return nullptr;
 }
 

@@ -52,7 +52,9 @@ class EvalOptions(ctypes.Structure):
                 ("long_waves_per_cu", ctypes.c_uint32), ("long_donate_every", ctypes.c_uint32),
                 ("long_fanout", ctypes.c_uint32), ("match_order", ctypes.c_uint32),
                 ("group_lanes", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
-                ("num_pushdown", ctypes.c_uint32), ("pushdown", ctypes.c_void_p)]
+                ("num_pushdown", ctypes.c_uint32), ("pushdown", ctypes.c_void_p),
+                ("steps_per_round", ctypes.c_uint32), ("refill_min", ctypes.c_uint32),
+                ("refill_max", ctypes.c_uint32), ("prune_batch", ctypes.c_uint32)]
 
 
 class PatternResult(ctypes.Structure):
@@ -458,7 +460,8 @@ def load_tuning(path):
     if host_lib().tfsmh_load_tuning(path.encode(), ctypes.byref(opt)) != 0:
         raise TfsmError(-1, host_lib().tfsmh_last_error().decode())
     names = ["grid_waves_per_cu", "split_steps", "split_fanout", "long_steps", "long_waves_per_cu",
-             "long_donate_every", "long_fanout", "match_order", "group_lanes"]
+             "long_donate_every", "long_fanout", "match_order", "group_lanes", "steps_per_round", "refill_min",
+             "refill_max", "prune_batch"]
     return {k: getattr(opt, k) for k in names if getattr(opt, k)}
 
 

@@ -56,6 +56,10 @@ CONFIGS = {
     # is the graph the learned order is tested on (not trained on).
     "rmat20_f5300_m5": ("rmat20", 4, 5300, 5, "serial", 1),
     "rmat20s5_f5300_m5": ("rmat20", 5, 5300, 5, "serial", 1),
+    # the same two from the parallel build (patterns of three or more
+    # vertices), which finishes in minutes where the serial one takes hours
+    "rmat20_f5300_m5_parallel": ("rmat20", 4, 5300, 5, "parallel", 2),
+    "rmat20s5_f5300_m5_parallel": ("rmat20", 5, 5300, 5, "parallel", 2),
     # Fraction-Score support (the reference's fraction-score build, which
     # prints like the parallel build; one thread keeps its output lines whole)
     "readme_frac_f1_m3": ("readme", 0, 1, 3, "fraction", 1),

@@ -70,3 +70,28 @@ def test_reference_binding_compiles_and_links(tmp_path):
                         "-L", tfsm.BUILD, "-ltfsm_gpu", "-Wl,-rpath," + tfsm.BUILD],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every struct the Python binding passes across the boundary has the
+    header's size and field offsets (a C program compiled against
+    include/tfsm_gpu.h prints them)."""
+    structs = {"tfsm_csr": tfsm.Csr, "tfsm_pattern": tfsm.Pattern, "tfsm_eval_options": tfsm.EvalOptions,
+               "tfsm_pattern_result": tfsm.PatternResult, "tfsm_call_stats": tfsm.CallStats,
+               "tfsm_ranker_weights": tfsm.RankerWeights}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "tfsm_gpu.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'    printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines.append("    return 0;\n}\n")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = str(tmp_path / "layout")
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(REPO, "include"), "-o", exe, str(src)], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert got[(cname, field)] == getattr(py, field).offset, (cname, field)

@@ -26,7 +26,9 @@ def test_shipped_tuning_files_load():
     files = glob.glob(os.path.join(tfsm.HERE, "tuning", "*.json"))
     for f in files:
         opts = tfsm.load_tuning(f)
-        assert set(opts) <= {"grid_waves_per_cu", "split_steps", "split_fanout", "long_steps"}
+        assert set(opts) <= {"grid_waves_per_cu", "split_steps", "split_fanout", "long_steps", "group_lanes",
+                             "steps_per_round", "refill_min", "refill_max", "prune_batch"}
+        assert opts.get("refill_min", 128) <= opts.get("refill_max", 256)
 
 
 @pytest.mark.parametrize("bad", [
