@@ -143,8 +143,32 @@ typedef struct {
     uint32_t steps_per_round;   /* DFS steps a lane takes between bookkeeping rounds (0: 8; at most 64) */
     uint32_t refill_min;        /* fewest and most lanes' worth of root tasks a wave takes from the */
     uint32_t refill_max;        /*   launch's counter at once (0: 128 and 256; 1 <= min <= max <= 4096) */
-    uint32_t prune_batch;       /* DP-iso passes launched between convergence checks (0: 6; at most 64) */
+    uint32_t prune_batch;       /* DP-iso passes launched between convergence checks (0: 6; at most 64);
+                                   a device-driven call runs prune_batch passes (0: 8)
+                                   and is redone host-driven when they do not reach the fixpoint */
+    uint32_t level_mode;        /* TFSM_LEVEL_* (0: device-driven with a captured hipGraph for
+                                   tfsm_eval_level, host-driven for tfsm_eval_pattern) */
 } tfsm_eval_options;
+
+/* How one call's launch sequence is driven.  None changes a result.
+ * Host-driven: the host reads the domain sizes back after the filter, makes
+ * the decisions and layout and launches the tables and the search.
+ * Device-driven: one launch sequence with no host round trip inside the
+ * call; the decisions, layout and task counts are computed on the device from
+ * regions sized by the candidates' sources, every kernel reads its arguments
+ * from a per-context descriptor, and with TFSM_LEVEL_GRAPH the sequence is
+ * captured once per context and option set in a hipGraph and replayed.  A
+ * call the device-driven path does not take (traced, Fraction-Score, patterns
+ * over 8 vertices or in the reference matching order, the wave-cooperative
+ * phase, more than 4096 patterns, regions over the device budget) runs
+ * host-driven; so does one whose filter needs more passes than the sequence
+ * holds (redone from the start). */
+enum {
+    TFSM_LEVEL_DEFAULT = 0,
+    TFSM_LEVEL_HOST = 1,
+    TFSM_LEVEL_DEVICE = 2,      /* device-driven, kernels launched one by one */
+    TFSM_LEVEL_GRAPH = 3        /* device-driven, one hipGraph launch */
+};
 
 /* decided_by */
 enum { TFSM_DECIDED_FILTER = 0, TFSM_DECIDED_EXPRESS = 1, TFSM_DECIDED_SEARCH = 2 };
@@ -251,6 +275,8 @@ typedef struct {
     uint32_t search_splits;     /* sub-tree pieces donated between lanes */
     uint32_t search_watchdog;   /* 1 when a drained wave left on its idle limit */
     float rank_ms;              /* features, scores and priority sort before the search */
+    uint32_t level_mode;        /* TFSM_LEVEL_* that ran the call (the last slice's) */
+    uint32_t filter_passes;     /* DP-iso passes that did work */
 } tfsm_call_stats;
 int tfsm_last_stats(tfsm_ctx *ctx, tfsm_call_stats *out);
 

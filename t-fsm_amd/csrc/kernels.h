@@ -106,6 +106,14 @@ struct PatStatic {
     uint16_t same_lab[kMaxV];
     uint32_t edge_pack[kCutEdges / 4];  // edges a | b << 4, one byte each, 0xff past the last
     uint32_t cut_ao[2 * kCutEdges];     // slot's row pairs at aux_region + cut_ao[slot]
+    // device-driven level calls (plan_pattern): a distinct-label acyclic
+    // pattern (decided by the filter's arc consistency), and where the
+    // pattern's table offsets and verdict bitmaps start in the call's regions
+    // (each sized by the pattern's sources, upper bounds of its domains)
+    uint32_t express;
+    uint32_t aux_base;                  // words into the table-offset region
+    uint32_t verdict_base;              // words into the verdict region
+    uint32_t aux_m0;                    // its first (pattern, slot) entry of the table work list
 };
 
 // Device-written state of one pattern.
@@ -117,7 +125,47 @@ struct PatState {
     uint32_t status;                    // 0 undecided, 1 frequent, 2 infrequent
     unsigned long long searches;
     uint32_t aux_total[kMaxSlots];      // entries of each slot's table
+    uint32_t decided;                   // TFSM_DECIDED_* (device-driven level calls)
+    uint32_t pad;
 };
+
+// Layout of one pattern once its domain sizes are known (reference
+// isCandidateSetValid and the express rule of comper.h:1619-1653, then the
+// sizes of buildTable's tables and of the verdict bitmaps): the decision,
+// and for a search-decided pattern its task prefix, each slot's row-pair
+// array at aux_region + aux_base (slot after slot, two words per row of
+// C(slot_a)) and its valid / invalid bitmaps at verdict_region +
+// verdict_base (vertex after vertex).  Device-driven level calls run it on
+// the device (plan_level, in the last compaction block) and repeat it on the
+// host after the call.
+// Returns the decision; `dom` holds the sizes.
+__host__ __device__ inline uint32_t plan_pattern(PatStatic &P, const uint32_t *dom, uint32_t *aux_region,
+                                                 uint32_t *verdict_region) {
+    uint32_t mn = 0xffffffffu;
+    for (uint32_t u = 0; u < P.nv; ++u) {
+        P.dom_size[u] = dom[u];
+        mn = dom[u] < mn ? dom[u] : mn;
+    }
+    if (P.express) return TFSM_DECIDED_EXPRESS;
+    if (mn < P.freq) return TFSM_DECIDED_FILTER;
+    P.task_prefix[0] = 0;
+    for (uint32_t u = 0; u < P.nv; ++u) P.task_prefix[u + 1] = P.task_prefix[u] + P.dom_size[u];
+    uint32_t ao = P.aux_base;
+    for (uint32_t sl = 0; sl < P.nslots; ++sl) {
+        if (sl < 2u * kCutEdges) P.cut_ao[sl] = ao;
+        P.aux_off[sl] = aux_region + ao;
+        ao += 2u * P.dom_size[P.slot_a[sl]];
+    }
+    uint32_t vo = P.verdict_base;
+    for (uint32_t u = 0; u < P.nv; ++u) {
+        const uint32_t wds = (P.dom_size[u] + 31u) / 32u;
+        P.valid_bits[u] = verdict_region + vo;
+        vo += wds;
+        P.invalid_bits[u] = verdict_region + vo;
+        vo += wds;
+    }
+    return TFSM_DECIDED_SEARCH;
+}
 
 // A push-down source: a retained child domain (sorted ids) and its bitmap
 // of proven-invalid entries.
@@ -188,9 +236,9 @@ struct SearchCounters {
 struct VerifyArgs {
     const PatStatic *ps;
     PatState *st;
-    const unsigned long long *task_base;   // P + 1
+    const unsigned long long *task_base;   // P + 1; task_base[P] is the number of tasks
     uint32_t P;
-    unsigned long long total;
+    uint32_t epoch;                        // record epoch (device-driven level calls)
     SearchCounters *ctr;
     const uint32_t *pool;
     const uint32_t *aux_region;            // every slot's row offsets of the call
@@ -220,6 +268,59 @@ struct VerifyArgs {
     uint32_t group_cap;                    // entries a launch may run in lane groups
     const uint32_t *hot;                   // with order: entries before *hot (of the first part)
                                            // form the hot tier, searched with the largest groups
+};
+
+// Everything the kernels of one device-driven level call read, in one
+// descriptor per context slot (c_level in kernels.inc).  The kernels of such
+// a call take only the slot (and a pass or part number), so their launch
+// sequence is the same for every level with the same options and can be
+// captured once in a hipGraph and replayed; the host writes the descriptor
+// before each launch.  Sizes the host cannot know before the filter (domain
+// sizes, task counts, table work) are computed on the device by
+// the last compaction block (plan_level) and read by the later kernels from
+// device memory.
+struct LevelArgs {
+    const CopyDesc *fills;                 // zero fills before the filter
+    uint32_t nfills;
+    uint32_t P;                            // patterns
+    DevGraph g;
+    PatStatic *ps;
+    PatState *st;                          // P + 1: the last entry's first word is the filter's change flag
+    // filter
+    WorkTable wsrc;                        // (pattern, u, source candidate)
+    const uint32_t *ex_off;                // push-down sources (null: none)
+    const PushSrc *ex;
+    uint32_t *pflags;                      // per DP-iso pass: 1 when it removed a candidate
+    uint32_t *chg;                         // three change-flag arrays of nchg words
+    uint32_t nchg;
+    uint32_t passes;                       // DP-iso passes of the call
+    const uint32_t *pairs_p, *pairs_u;     // compaction: (pattern, u) pairs
+    uint32_t npairs;
+    uint32_t *passes_run;                  // passes that did work (counted by the passes themselves)
+    uint32_t *blocks_done;                 // compaction blocks finished (the last one plans the level)
+    // layout (plan_level)
+    uint32_t *aux_region;
+    uint32_t *verdict_region;
+    unsigned long long *task_base;         // P + 1
+    unsigned long long *aux_base;          // aux_M + 1: the table work list's ranges
+    const uint32_t *aux_pat, *aux_x;       // aux_M: (pattern, slot) of every slot of every pattern
+    uint32_t aux_M;
+    // tables
+    uint32_t *pool, *pool_used, *overflow;
+    uint32_t pool_cap;
+    // learned order
+    const uint16_t *W;
+    float *logits, *sums, *qsums, *ssums;
+    uint32_t *keys, *pu, *hist, *tiles, *order, *order_pu;
+    uint32_t nhist, ntiles;
+    // existence search: the first launch and the deferred one
+    VerifyArgs va[2];
+    // phase clock (the device's constant-rate wall clock): [0] the first
+    // block's start (min), [1] the end of the filter and the plan, [2] of the
+    // tables (max),
+    // [3] the first search block's start (min), [4] the last search wave's end
+    // (max); the host uploads the starting values with the call
+    unsigned long long *clk;
 };
 
 }  // namespace tfsm_dev

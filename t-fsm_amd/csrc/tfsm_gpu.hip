@@ -1,13 +1,20 @@
 // libtfsm_gpu.so: the C ABI of include/tfsm_gpu.h on MI355X (gfx950).
 //
 // One translation unit: the kernels (kernels.inc) and the host side that
-// plans, launches and reads back an evaluation.  The host side follows one
-// pattern (tfsm_eval_pattern) or one extension level (tfsm_eval_level)
-// through the same launch sequence:
-//   LDF -> DP-iso passes until a pass removes nothing -> compaction
-//   -> (host) sizes, express / filter decisions, matching plans
-//   -> table count -> per-slot scan + pool allocation -> table fill
-//   -> existence search -> (host) verdicts and retained child domains.
+// plans, launches and reads back an evaluation of one pattern
+// (tfsm_eval_pattern) or one extension level (tfsm_eval_level).  Two drivers
+// run the same kernels:
+//   host-driven:   LDF -> DP-iso passes until a pass removes nothing ->
+//                  compaction -> (host) sizes, express / filter decisions,
+//                  matching plans, layout -> one-pass table build ->
+//                  learned order -> existence search -> (host) verdicts and
+//                  retained child domains;
+//   device-driven: the same phases in one launch sequence with no host round
+//                  trip inside it (the last compaction block makes the
+//                  decisions and layout on the device), every kernel reading
+//                  a per-context descriptor, so that the sequence is captured
+//                  once in a hipGraph and replayed for every level of the
+//                  same shape; the host reads the results back once.
 #include "kernels.inc"
 
 #include <algorithm>
@@ -16,6 +23,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -211,6 +219,18 @@ constexpr uint32_t kDefaultStepsPerRound = 8;
 constexpr uint32_t kDefaultRefillMin = 128;
 constexpr uint32_t kDefaultRefillMax = 256;
 constexpr uint32_t kPrioBlocks = 512;        // most blocks of the priority sort (one histogram column each)
+// device-driven calls: blocks of the priority sort (fixed, so that the launch
+// sequence does not depend on the task count), most patterns, and the most
+// device memory their regions (sized by the candidates' sources) may take
+constexpr uint32_t kGraphPrioBlocks = 512;
+constexpr uint32_t kDeviceMaxPatterns = 4096;
+constexpr size_t kDeviceRegionBudget = size_t(4) << 30;
+constexpr uint32_t kNoSlot = 0xffffffffu;
+// launch grids of a device-driven call, which with the options make a graph's key
+struct LevelShape {
+    uint32_t learned, passes, gfill, gsrc, gpair, gaux, gsample, grank, vgrid;
+    bool operator==(const LevelShape &o) const { return std::memcmp(this, &o, sizeof(o)) == 0; }
+};
 
 }  // namespace
 
@@ -263,7 +283,37 @@ struct tfsm_ctx {
     hipEvent_t ev_sync = nullptr;           // end of the stream's work, polled by sync_spin
     tfsm_call_stats stats{};
     std::string err;
+    // device-driven level calls
+    uint32_t level_slot = kNoSlot;          // this context's descriptor in c_level
+    LevelArgs *d_level = nullptr;           // its device address
+    Pinned<LevelArgs> h_level;              // the descriptor's staging
+    Pinned<unsigned long long> h_clk;       // the phase clock read back
+    int wall_khz = 100000;                  // rate of the device's wall clock
+    std::vector<std::pair<LevelShape, hipGraphExec_t>> graphs;   // captured launch sequences
 };
+
+namespace {
+// c_level slots in use, per device
+std::mutex g_slot_mutex;
+std::map<int, uint32_t> g_slots_used;
+
+uint32_t acquire_slot(int device) {
+    std::lock_guard<std::mutex> lk(g_slot_mutex);
+    uint32_t &m = g_slots_used[device];
+    for (uint32_t i = 0; i < kLevelSlots; ++i)
+        if (!(m >> i & 1u)) {
+            m |= 1u << i;
+            return i;
+        }
+    return kNoSlot;
+}
+
+void release_slot(int device, uint32_t slot) {
+    if (slot == kNoSlot) return;
+    std::lock_guard<std::mutex> lk(g_slot_mutex);
+    g_slots_used[device] &= ~(1u << slot);
+}
+}  // namespace
 
 namespace {
 
@@ -475,6 +525,8 @@ struct EvalCall {
     uint32_t freq;
     tfsm_eval_options opt;
     tfsm_pattern_result *out;
+    bool level_call = false;        // from tfsm_eval_level (the default driver differs)
+    bool force_host = false;        // redo of a device-driven call whose filter did not converge
 };
 
 // Waits for the stream by polling an event.  A blocking wait adds tens of
@@ -522,6 +574,500 @@ double frac_value(const FracSum &f) {
     return std::ldexp((double)x, -96);
 }
 
+// Stats, per-pattern results and retained child domains of a finished call,
+// from the states, counters and flags read back (both drivers).
+struct Finished {
+    std::vector<PatStatic> &hs;
+    const PatState *hst;
+    const SearchCounters *hctr;
+    const uint32_t *hpool;          // pool used, pool overflow, filter passes run
+    const std::vector<int> &decided;
+    const FracSum *hfrac;
+    unsigned long long total_tasks;
+    uint32_t verify_launches;
+    uint32_t level_mode;
+    bool timed;                     // the stream events of every phase were recorded
+};
+
+int finish_call(tfsm_ctx *c, const EvalCall &call, const Finished &f) {
+    const uint32_t n = call.n;
+    const tfsm_eval_options &opt = call.opt;
+    const bool fraction = opt.mode == TFSM_MODE_FRACTION;
+    std::vector<PatStatic> &hs = f.hs;
+    const PatState *hst = f.hst;
+    const SearchCounters *hctr = f.hctr;
+    const uint32_t *hpool = f.hpool;
+    const std::vector<int> &decided = f.decided;
+    const FracSum *hfrac = f.hfrac;
+    const unsigned long long total_tasks = f.total_tasks;
+    const uint32_t verify_launches = f.verify_launches;
+    hipStream_t s = c->stream;
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    c->stats.level_mode = f.level_mode;
+    c->stats.filter_passes = hpool[2];
+    if (f.timed) {
+        HIPX(hipEventElapsedTime(&c->stats.filter_ms, c->ev[0], c->ev[1]));
+        HIPX(hipEventElapsedTime(&c->stats.table_ms, c->ev[1], c->ev[2]));
+    }
+    if (total_tasks) {
+        if (f.timed) {
+            HIPX(hipEventElapsedTime(&c->stats.search_ms, c->ev[4], c->ev[5]));
+            HIPX(hipEventElapsedTime(&c->stats.rank_ms, c->ev[2], c->ev[4]));
+        }
+        c->stats.search_launches = verify_launches;
+        c->stats.search_splits = std::min(hctr->split_tail, kSplitCap);
+        c->stats.search_watchdog = hctr->watchdog ? 1u : 0u;
+        if (verify_launches == 2) {
+            c->stats.search_splits += std::min(hctr[1].split_tail, kSplitCap);
+            if (hctr[1].watchdog) c->stats.search_watchdog = 1;
+        }
+    }
+    {
+        // algorithmic bytes (definitions in tfsm_gpu.h)
+        // the pool reservation is the neighbour ids the table build scanned
+        unsigned long long tb = 4ull * hpool[0], sb = 0;
+        for (uint32_t p = 0; p < n; ++p) {
+            if (decided[p] != TFSM_DECIDED_SEARCH) continue;
+            const PatStatic &P = hs[p];
+            const PatState &S = hst[p];
+            for (uint32_t sl = 0; sl < P.nslots; ++sl) {
+                const unsigned long long rows = P.dom_size[P.slot_a[sl]];
+                tb += 16ull * rows + 4ull * S.aux_total[sl];
+            }
+            for (uint32_t u = 0; u < P.nv; ++u) {
+                if (!P.dom_size[u]) continue;
+                double per = 0;
+                for (uint32_t sl = 0; sl < P.nslots; ++sl)
+                    if (P.slot_a[sl] == u) per += 8.0 + 4.0 * (double)S.aux_total[sl] / (double)P.dom_size[u];
+                sb += (unsigned long long)(per * (double)(S.valid_cnt[u] + S.invalid_cnt[u]));
+            }
+        }
+        c->stats.table_bytes = tb;
+        c->stats.search_bytes = sb;
+    }
+
+    // ---------------- results and retained child domains ----------------
+    std::vector<CopyDesc> keep;                 // the retained domains' copies, one launch
+    for (uint32_t p = 0; p < n; ++p) {
+        const PatStatic &P = hs[p];
+        const PatState &S = hst[p];
+        tfsm_pattern_result &r = call.out[p];
+        std::memset(&r, 0, sizeof(r));
+        r.decided_by = decided[p];
+        uint32_t lo = 0xffffffffu, hi = 0xffffffffu;
+        for (uint32_t u = 0; u < P.nv; ++u) {
+            r.domain_size[u] = P.dom_size[u];
+            if (decided[p] == TFSM_DECIDED_EXPRESS) {
+                r.valid_count[u] = P.dom_size[u];
+            } else if (decided[p] == TFSM_DECIDED_SEARCH) {
+                r.valid_count[u] = S.valid_cnt[u];
+                r.invalid_count[u] = S.invalid_cnt[u];
+            }
+            lo = std::min(lo, r.valid_count[u]);
+            hi = std::min(hi, r.domain_size[u] - r.invalid_count[u]);
+            r.verdicts += r.valid_count[u] + r.invalid_count[u];
+        }
+        r.support_lower = lo;
+        r.support_upper = hi;
+        r.searches = S.searches;
+        if (fraction) {
+            // a pattern stopped by the valid-count bound (status 2) is
+            // infrequent; every other search- or express-decided one has all
+            // its verdicts, so its sums are exact
+            double mn = 0;
+            for (uint32_t u = 0; u < P.nv; ++u) {
+                r.fraction[u] = frac_value(hfrac[(size_t)p * kMaxV + u]);
+                mn = u == 0 ? r.fraction[u] : std::min(mn, r.fraction[u]);
+            }
+            r.fraction_support = mn;
+            r.frequent = decided[p] != TFSM_DECIDED_FILTER && !(decided[p] == TFSM_DECIDED_SEARCH && S.status == 2) &&
+                         mn >= (double)call.freq;
+        } else if (decided[p] == TFSM_DECIDED_SEARCH && opt.mode == TFSM_MODE_CAPPED && S.status != 0) {
+            r.frequent = S.status == 1;
+        } else {
+            r.frequent = decided[p] != TFSM_DECIDED_FILTER && lo >= call.freq;
+        }
+        if (r.frequent && opt.retain_domains) {
+            size_t words = 0;
+            for (uint32_t u = 0; u < P.nv; ++u) words += P.dom_size[u] + (P.dom_size[u] + 31) / 32;
+            Retained R;
+            R.nv = P.nv;
+            for (uint32_t u = 0; u < P.nv; ++u) R.vlab[u] = P.vlab[u];
+            R.buf = c->cache.get(std::max<size_t>(words, 1) * sizeof(uint32_t), &R.cls);
+            uint32_t *b = static_cast<uint32_t *>(R.buf);
+            for (uint32_t u = 0; u < P.nv; ++u) {
+                const size_t wds = (P.dom_size[u] + 31) / 32;
+                R.size[u] = P.dom_size[u];
+                R.dom[u] = b; b += P.dom_size[u];
+                R.invalid[u] = b; b += wds;
+                if (P.dom_size[u]) keep.push_back({P.dom[u], R.dom[u], P.dom_size[u]});
+                if (wds) keep.push_back({decided[p] == TFSM_DECIDED_SEARCH ? P.invalid_bits[u] : nullptr, R.invalid[u], wds});
+            }
+            r.domain_handle = c->next_handle++;
+            c->retained.emplace(r.domain_handle, R);
+        }
+    }
+    if (!keep.empty()) {
+        const CopyDesc *d_keep = upload(c, keep);
+        hipLaunchKernelGGL(k_copy_batch, dim3((uint32_t)std::min<size_t>(keep.size(), 4096)), dim3(kBlock), 0, s,
+                           d_keep, (uint32_t)keep.size());
+        HIPX(hipGetLastError());
+    }
+    c->last_static = hs;
+    c->last_state.assign(hst, hst + n);
+    c->last_pool_used = hpool[0];
+    c->last_decided = decided;
+    // the retained-domain copies stay queued: the next call waits for the stream before it
+    // reuses the staging, and every function that reads device memory from the host syncs first
+    return TFSM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Device-driven calls (TFSM_LEVEL_DEVICE / TFSM_LEVEL_GRAPH)
+// ---------------------------------------------------------------------------
+constexpr int kNotTaken = 1000;     // not eligible: the host-driven path runs the call
+constexpr int kRedoHost = 1001;     // the filter needed more passes: redo host-driven
+constexpr int kRedoDevice = 1002;   // the pool was grown: redo the call
+
+// Blocks for `items` work items of one thread each, rounded up to a power of
+// two (so that levels of similar size share a captured sequence), at most cap.
+uint32_t grid_class(unsigned long long items, uint32_t cap) {
+    const unsigned long long blocks = (items + kBlock - 1) / kBlock;
+    uint32_t g = 1;
+    while (g < blocks && g < cap) g <<= 1;
+    return std::min(g, cap);
+}
+
+constexpr uint32_t kGraphScanTiles = ((uint32_t)kPrioBins * kGraphPrioBlocks + kScanTile - 1) / kScanTile;
+static_assert(kGraphScanTiles <= 4 * kBlock, "the tile sums fit the one-block scan");
+
+// The launch sequence of a device-driven call.  Every kernel reads the
+// context's descriptor, so the sequence depends only on the shape; with
+// `events` the phase events are recorded between the kernels.
+void launch_level(tfsm_ctx *c, const LevelShape &sh, bool events) {
+    hipStream_t s = c->stream;
+    const uint32_t slot = c->level_slot;
+    if (events) HIPX(hipEventRecord(c->ev[0], s));
+    hipLaunchKernelGGL(k_fills_g, dim3(sh.gfill), dim3(kBlock), 0, s, slot);
+    hipLaunchKernelGGL(k_ldf_g, dim3(sh.gsrc), dim3(kBlock), 0, s, slot);
+    for (uint32_t i = 0; i < sh.passes; ++i) hipLaunchKernelGGL(k_prune_g, dim3(sh.gsrc), dim3(kBlock), 0, s, slot, i);
+    hipLaunchKernelGGL(k_compact_g, dim3(sh.gpair), dim3(kBlock), 0, s, slot);    // and the plan
+    if (events) HIPX(hipEventRecord(c->ev[1], s));
+    hipLaunchKernelGGL(k_aux_build_g, dim3(sh.gaux), dim3(kBlock), 0, s, slot);
+    if (events) HIPX(hipEventRecord(c->ev[2], s));
+    if (sh.learned) {
+        hipLaunchKernelGGL(k_rank_scores_g, dim3(sh.gsample), dim3(kBlock), 0, s, slot, 1u);
+        hipLaunchKernelGGL(k_rank_scores_g, dim3(sh.grank), dim3(kBlock), 0, s, slot, 0u);
+        hipLaunchKernelGGL(k_priority_g, dim3(kGraphPrioBlocks), dim3(kBlock), 0, s, slot);
+        hipLaunchKernelGGL(k_scan_g, dim3(kGraphScanTiles), dim3(kBlock), 0, s, slot, 0u);
+        hipLaunchKernelGGL(k_scan_g, dim3(1), dim3(kBlock), 0, s, slot, 1u);
+        hipLaunchKernelGGL(k_scan_g, dim3(kGraphScanTiles), dim3(kBlock), 0, s, slot, 2u);
+        hipLaunchKernelGGL(k_prio_scatter_g, dim3(kGraphPrioBlocks), dim3(kBlock), 0, s, slot);
+    }
+    if (events) HIPX(hipEventRecord(c->ev[4], s));
+    hipLaunchKernelGGL((k_verify_g<8, tfsm_lane::kOrderAdaptive>), dim3(sh.vgrid), dim3(kBlock), 0, s, slot, 0u);
+    if (sh.learned)
+        hipLaunchKernelGGL((k_verify_g<8, tfsm_lane::kOrderAdaptive>), dim3(sh.vgrid), dim3(kBlock), 0, s, slot, 1u);
+    if (events) {
+        HIPX(hipEventRecord(c->ev[5], s));
+        HIPX(hipEventRecord(c->ev[3], s));
+    }
+    HIPX(hipGetLastError());
+}
+
+// The context's captured sequence for a shape: captured and instantiated on
+// first use, then replayed.
+hipGraphExec_t level_graph(tfsm_ctx *c, const LevelShape &sh) {
+    for (auto &g : c->graphs)
+        if (g.first == sh) return g.second;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    HIPX(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    bool launched = true;
+    try {
+        launch_level(c, sh, false);
+    } catch (const HipError &) {
+        launched = false;
+    }
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (launched && e == hipSuccess && graph) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!launched || e != hipSuccess || !exec) {
+        (void)hipGetLastError();
+        throw HipError{e == hipSuccess ? hipErrorUnknown : e, "capture of the level sequence"};
+    }
+    if (c->graphs.size() >= 16) {
+        (void)hipGraphExecDestroy(c->graphs.front().second);
+        c->graphs.erase(c->graphs.begin());
+    }
+    c->graphs.emplace_back(sh, exec);
+    return exec;
+}
+
+// One call with no host round trip inside it: the filter, the device-side
+// decisions and layout (plan_level), the tables, the learned order and the
+// search, in one launch sequence over regions sized by the candidates'
+// sources; the host reads the states and counters back once at the end.
+int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, const uint32_t *slot_of,
+               const std::vector<char> &distinct, const std::vector<char> &acyclic, uint32_t *bits_region,
+               size_t bits_words, PatState *d_st, const std::vector<uint32_t> &ex_off, const std::vector<PushSrc> &ex,
+               int attempt) {
+    const uint32_t n = call.n;
+    const tfsm_eval_options &opt = call.opt;
+    const uint32_t lm = opt.level_mode == TFSM_LEVEL_DEFAULT
+                            ? (call.level_call ? (uint32_t)TFSM_LEVEL_GRAPH : (uint32_t)TFSM_LEVEL_HOST)
+                            : opt.level_mode;
+    if (lm != TFSM_LEVEL_DEVICE && lm != TFSM_LEVEL_GRAPH) return kNotTaken;
+    const bool learned = opt.task_order == TFSM_ORDER_LEARNED;
+    const bool wave_phase = (opt.long_steps == 0 ? kDefaultLongSteps : opt.long_steps) != 0xffffffffu;
+    if (c->level_slot == kNoSlot || opt.trace || opt.mode == TFSM_MODE_FRACTION || wave_phase ||
+        n > kDeviceMaxPatterns)
+        return kNotTaken;
+    // regions sized by the sources (every domain is at most its source)
+    unsigned long long aux_ub = 0, ver_ub = 0, task_ub = 0, units_ub = 0, M = 0;
+    uint32_t max_edges = 0;
+    for (uint32_t p = 0; p < n; ++p) {
+        PatStatic &P = hs[p];
+        if (!P.adaptive || P.nv > 8) return kNotTaken;
+        P.express = distinct[p] && acyclic[p];
+        P.aux_base = (uint32_t)std::min<unsigned long long>(aux_ub, 0xffffffffull);
+        P.verdict_base = (uint32_t)std::min<unsigned long long>(ver_ub, 0xffffffffull);
+        P.aux_m0 = (uint32_t)M;
+        for (uint32_t sl = 0; sl < P.nslots; ++sl) {
+            aux_ub += 2ull * P.src_n[P.slot_a[sl]];
+            units_ub += (P.src_n[P.slot_a[sl]] + 63ull) / 64 * 64;
+        }
+        for (uint32_t u = 0; u < P.nv; ++u) {
+            ver_ub += 2ull * ((P.src_n[u] + 31ull) / 32);
+            task_ub += P.src_n[u];
+        }
+        M += P.nslots;
+        max_edges = std::max(max_edges, P.nslots / 2);
+    }
+    const size_t region_bytes = 4 * (aux_ub + ver_ub) + (learned ? 20 * task_ub : 0);
+    if (aux_ub >= (1ull << 31) || ver_ub >= (1ull << 31) || task_ub >= (1ull << 31) || units_ub >= (1ull << 31) ||
+        region_bytes > kDeviceRegionBudget)
+        return kNotTaken;
+    hipStream_t s = c->stream;
+    // DP-iso passes of the sequence: prune_batch, or a margin over the
+    // host-driven batch by default; a level that needs more is redone
+    // host-driven (a pass after the fixpoint returns at once)
+    const uint32_t passes = opt.prune_batch ? opt.prune_batch : kPruneBatch + 2;
+    c->trace_len = 0;
+
+    // regions and work lists
+    uint32_t *aux_region = c->work.alloc<uint32_t>(std::max<unsigned long long>(aux_ub, 1));
+    uint32_t *verdict_region = c->work.alloc<uint32_t>(std::max<unsigned long long>(ver_ub, 1));
+    unsigned long long *task_base = c->work.alloc<unsigned long long>(n + 1);
+    unsigned long long *aux_base = c->work.alloc<unsigned long long>(M + 1);
+    const uint32_t nchg = n * kMaxV;
+    uint32_t *d_chg = c->work.alloc<uint32_t>(3 * (size_t)nchg);
+    HostWork src_work;
+    std::vector<uint32_t> pair_p, pair_u, aux_pat, aux_x;
+    for (uint32_t p = 0; p < n; ++p) {
+        for (uint32_t u = 0; u < hs[p].nv; ++u) {
+            src_work.add(p, u, hs[p].src_n[u]);
+            pair_p.push_back(p);
+            pair_u.push_back(u);
+        }
+        for (uint32_t sl = 0; sl < hs[p].nslots; ++sl) {
+            aux_pat.push_back(p);
+            aux_x.push_back(sl);
+        }
+    }
+    LevelArgs *L = c->h_level.ensure(1);
+    std::memset(L, 0, sizeof(*L));
+    const uint32_t group_max = opt.group_lanes == 0 ? kDefaultGroupLanes : opt.group_lanes;
+    const size_t gz = std::min<unsigned long long>(task_ub, kGroupCap);
+    PhaseBatch b(c, n * sizeof(PatStatic) + src_work.bytes() + 4 * ex_off.size() + sizeof(PushSrc) * ex.size() +
+                        8 * pair_p.size() + 8 * aux_pat.size() + 6 * 256 +
+                        sizeof(CopyDesc) * ((bits_words + ver_ub) / kFillPiece + (n + 1) * sizeof(PatState) / 4 / kFillPiece +
+                                            (size_t)n * kMaxV * (2 + kPrioQ) / kFillPiece + 2 * gz / kFillPiece + 24) +
+                        256);
+    L->ps = b.add(hs.data(), n);
+    L->wsrc = src_work.to_device(b);
+    L->ex_off = ex.empty() ? nullptr : b.add(ex_off);
+    L->ex = ex.empty() ? nullptr : b.add(ex);
+    L->pairs_p = b.add(pair_p);
+    L->pairs_u = b.add(pair_u);
+    L->aux_pat = b.add(aux_pat);
+    L->aux_x = b.add(aux_x);
+    b.zero(bits_region, bits_words * sizeof(uint32_t));
+    b.zero(d_st, (n + 1) * sizeof(PatState));
+    // pool used, pool overflow, filter passes run, compaction blocks done
+    b.zero(c->d_flags, 4 * sizeof(uint32_t));
+    b.zero(c->d_pflags, passes * sizeof(uint32_t));
+    b.zero(d_chg, 3 * (size_t)nchg * sizeof(uint32_t));
+    if (ver_ub) b.zero(verdict_region, ver_ub * sizeof(uint32_t));
+    b.zero(c->d_ctr, 2 * sizeof(SearchCounters));
+    if (group_max > 1 && gz) {
+        b.zero(c->d_piece_cnt + kSplitCap, gz * sizeof(uint32_t));
+        b.zero(c->d_piece_cnt + kSplitCap + kGroupCap, gz * sizeof(uint32_t));
+    }
+    float *rank_sums = nullptr;
+    if (learned) {
+        rank_sums = c->work.alloc<float>((size_t)n * kMaxV * (2 + kPrioQ));
+        b.zero(rank_sums, (size_t)n * kMaxV * (2 + kPrioQ) * sizeof(float));
+    }
+    L->nfills = (uint32_t)b.zeros.size();
+    L->fills = b.add(b.zeros);
+    b.zeros.clear();
+    // the phase clock's starting values (min slots at the largest value)
+    const std::vector<unsigned long long> clk0{~0ull, 0ull, 0ull, ~0ull, 0ull};
+    L->clk = b.add(clk0);
+    L->P = n;
+    L->g = c->g;
+    L->st = d_st;
+    L->pflags = c->d_pflags;
+    L->chg = d_chg;
+    L->nchg = nchg;
+    L->passes = passes;
+    L->npairs = (uint32_t)pair_p.size();
+    L->passes_run = c->d_flags + 2;
+    L->blocks_done = c->d_flags + 3;
+    L->aux_region = aux_region;
+    L->verdict_region = verdict_region;
+    L->task_base = task_base;
+    L->aux_base = aux_base;
+    L->aux_M = (uint32_t)M;
+    L->pool = c->d_pool;
+    L->pool_used = c->d_flags;
+    L->overflow = c->d_flags + 1;
+    L->pool_cap = c->pool_cap;
+    uint32_t *hist = nullptr;
+    if (learned) {
+        L->W = c->d_rank_w;
+        L->logits = c->work.alloc<float>(std::max<unsigned long long>(task_ub, 1));
+        L->sums = rank_sums;
+        L->qsums = rank_sums + (size_t)n * kMaxV;
+        L->ssums = L->qsums + (size_t)n * kMaxV * kPrioQ;
+        L->keys = c->work.alloc<uint32_t>(std::max<unsigned long long>(task_ub, 1));
+        L->pu = c->work.alloc<uint32_t>(std::max<unsigned long long>(task_ub, 1));
+        L->order = c->work.alloc<uint32_t>(std::max<unsigned long long>(task_ub, 1));
+        L->order_pu = c->work.alloc<uint32_t>(std::max<unsigned long long>(task_ub, 1));
+        L->nhist = kPrioBins * kGraphPrioBlocks;
+        L->ntiles = kGraphScanTiles;
+        hist = c->work.alloc<uint32_t>(L->nhist);
+        L->hist = hist;
+        L->tiles = c->work.alloc<uint32_t>(kGraphScanTiles);
+    }
+    // the search launches (the same settings as the host-driven sequence)
+    const uint32_t per_cu = opt.grid_waves_per_cu
+                                ? std::min((uint32_t)c->verify_blocks8a, std::max(1u, opt.grid_waves_per_cu / (kBlock / 64)))
+                                : (uint32_t)c->verify_blocks8a;
+    for (uint32_t part = 0; part < 2; ++part) {
+        VerifyArgs &va = L->va[part];
+        va.ps = L->ps;
+        va.st = d_st;
+        va.task_base = task_base;
+        va.P = n;
+        va.epoch = ++c->epoch;
+        va.ctr = c->d_ctr + part;
+        va.pool = c->d_pool;
+        va.aux_region = aux_region;
+        va.max_edges = max_edges;
+        va.overflow = c->d_flags + 1;
+        va.steps_per_round = opt.steps_per_round ? opt.steps_per_round : kDefaultStepsPerRound;
+        va.refill_min = opt.refill_min ? opt.refill_min : kDefaultRefillMin;
+        va.refill_max = opt.refill_max ? opt.refill_max : kDefaultRefillMax;
+        va.split_steps = opt.split_steps == 0 ? kDefaultSplitSteps : opt.split_steps;
+        if (opt.split_steps == 0xffffffffu) va.split_steps = 0;
+        va.split_fanout = opt.split_fanout == 0 ? kDefaultSplitFanout : std::min(opt.split_fanout, 64u);
+        va.records = c->d_records;
+        va.split_cap = kSplitCap;
+        va.piece_cnt = c->d_piece_cnt;
+        va.piece_incomplete = c->d_piece_inc;
+        va.group_max = group_max;
+        va.group_base = kSplitCap + part * kGroupCap;
+        va.group_cap = kGroupCap;
+        va.long_steps = 0xffffffffu;
+        if (learned) {
+            va.order = L->order;
+            va.order_pu = L->order_pu;
+            va.abort_stopped = 1;
+            va.part = part + 1;
+            // after the scan, the first deferred bin of block 0 holds the
+            // number of first-part entries; the first key after the hot tier,
+            // the hot tier's size
+            va.split = hist + (size_t)kDeferredKey * kGraphPrioBlocks;
+            va.hot = hist + (size_t)kHotKeys * kGraphPrioBlocks;
+        }
+    }
+    LevelShape sh;
+    std::memset(&sh, 0, sizeof(sh));
+    sh.learned = learned;
+    sh.passes = passes;
+    sh.gfill = grid_class((unsigned long long)L->nfills * kBlock, 4096);
+    sh.gsrc = grid_class(src_work.base.back(), (uint32_t)c->num_cus * 16);
+    sh.gpair = grid_class((unsigned long long)pair_p.size() * kBlock, (uint32_t)c->num_cus * 16);
+    // the table build's waves loop over the work (the sizes are known only on
+    // the device): two blocks per CU; more resident waves than that made the
+    // build slower (its scattered loads contend in the L1)
+    sh.gaux = grid_class(units_ub, (uint32_t)c->num_cus * 2);
+    // scoring: every block first loads the weights into LDS, so the waves
+    // loop over the tasks with eight blocks per CU at most
+    sh.gsample = grid_class((task_ub + kSample - 1) / kSample, (uint32_t)c->num_cus * 8);
+    sh.grank = grid_class(task_ub, (uint32_t)c->num_cus * 8);
+    sh.vgrid = (uint32_t)c->num_cus * per_cu;
+    // the batch's uploads, then the descriptor, then the sequence
+    b.flush();
+    HIPX(hipMemcpyAsync(c->d_level, L, sizeof(LevelArgs), hipMemcpyHostToDevice, s));
+    // stream events between the kernels (kernel by kernel) or the phase
+    // clock the kernels keep (a graph: events recorded inside a captured
+    // sequence carry no times the host can read here)
+    const bool graph = lm == TFSM_LEVEL_GRAPH;
+    if (graph) HIPX(hipGraphLaunch(level_graph(c, sh), s));
+    else launch_level(c, sh, true);
+    // one readback: the states (with the filter's change flag), then the
+    // counters of both search launches and the pool and pass words
+    PatState *hst = c->h_state.ensure(n + 1);
+    HIPX(hipMemcpyAsync(hst, d_st, (n + 1) * sizeof(PatState), hipMemcpyDeviceToHost, s));
+    SearchCounters *hctr = c->h_ctr.ensure(3);
+    HIPX(hipMemcpyAsync(hctr, c->d_ctr, 2 * sizeof(SearchCounters) + 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    const uint32_t *hpool = reinterpret_cast<const uint32_t *>(hctr + 2);
+    unsigned long long *hclk = c->h_clk.ensure(5);
+    if (graph) HIPX(hipMemcpyAsync(hclk, L->clk, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    sync_spin(c);
+    if (*reinterpret_cast<const uint32_t *>(hst + n)) return kRedoHost;
+    if (hpool[1]) {
+        // the tables did not fit: grow the pool to twice the measured need and redo the call
+        if (attempt >= 4) throw ArgError{TFSM_ERR_LIMIT, "table pool overflow"};
+        const unsigned long long need = (unsigned long long)hpool[0] * 2;
+        if (need > 0xfffffff0ull) throw ArgError{TFSM_ERR_LIMIT, "tables exceed 2^32 entries"};
+        HIPX(hipFree(c->d_pool));
+        c->d_pool = nullptr;
+        c->pool_cap = (uint32_t)std::max<unsigned long long>(need, c->pool_cap);
+        HIPX(hipMalloc(&c->d_pool, (size_t)c->pool_cap * sizeof(uint32_t)));
+        return kRedoDevice;
+    }
+    // the device's layout, repeated on the host for the results, the
+    // retained domains and the inspection functions
+    std::vector<int> decided(n);
+    unsigned long long total_tasks = 0;
+    for (uint32_t p = 0; p < n; ++p) {
+        decided[p] = (int)plan_pattern(hs[p], hst[p].dom_size, aux_region, verdict_region);
+        if ((uint32_t)decided[p] != hst[p].decided) throw ArgError{TFSM_ERR_HIP, "device layout differs from the host's"};
+        if (decided[p] == TFSM_DECIDED_SEARCH) {
+            if (n == 1) make_plans(hs[p], slot_of + (size_t)p * kMaxV * kMaxV);
+            total_tasks += hs[p].task_prefix[hs[p].nv];
+        }
+    }
+    const int rc = finish_call(c, call, Finished{hs, hst, hctr, hpool, decided, nullptr, total_tasks,
+                                                 learned ? 2u : 1u, lm, !graph});
+    if (graph) {
+        // the phase clock in ms: first block .. compaction .. tables .. search start .. end
+        const double per_ms = (double)c->wall_khz;
+        auto span = [&](int a, int b) { return hclk[b] > hclk[a] ? (float)((double)(hclk[b] - hclk[a]) / per_ms) : 0.f; };
+        c->stats.filter_ms = span(0, 1);
+        c->stats.table_ms = span(1, 2);
+        if (total_tasks) {
+            c->stats.rank_ms = span(2, 3);
+            c->stats.search_ms = span(3, 4);
+        }
+    }
+    return rc;
+}
+
 int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     const uint32_t n = call.n;
     const tfsm_eval_options &opt = call.opt;
@@ -542,6 +1088,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         if (lo > hi || hi > 4096) throw ArgError{TFSM_ERR_ARGUMENT, "refill_min <= refill_max <= 4096 required"};
     }
     if (opt.prune_batch > 64) throw ArgError{TFSM_ERR_ARGUMENT, "prune_batch must be at most 64"};
+    if (opt.level_mode > TFSM_LEVEL_GRAPH) throw ArgError{TFSM_ERR_ARGUMENT, "unknown level mode"};
     const uint32_t prune_batch = opt.prune_batch ? opt.prune_batch : kPruneBatch;
     const bool learned = opt.task_order == TFSM_ORDER_LEARNED;
     if (learned && !c->ranker_installed)
@@ -688,6 +1235,19 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         }
     }
 
+    // ---------------- device-driven calls ----------------
+    if (!call.force_host) {
+        const int r = run_device(c, call, hs, slot_of.data(), distinct, acyclic, bits_region, bits_words, d_st, ex_off,
+                                 ex, attempt);
+        if (r == kRedoHost) {
+            EvalCall h = call;
+            h.force_host = true;
+            return run_eval(c, h, 0);
+        }
+        if (r == kRedoDevice) return run_eval(c, call, attempt + 1);
+        if (r != kNotTaken) return r;
+    }
+
     // ---------------- filter: LDF, DP-iso to the fixpoint, compaction ----------------
     HostWork src_work;
     std::vector<uint32_t> pair_p, pair_u;
@@ -709,7 +1269,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     const uint32_t *d_pair_p = b1.add(pair_p), *d_pair_u = b1.add(pair_u);
     b1.zero(bits_region, bits_words * sizeof(uint32_t));
     b1.zero(d_st, (n + 1) * sizeof(PatState));
-    b1.zero(c->d_flags, 2 * sizeof(uint32_t));        // pool used, pool overflow
+    b1.zero(c->d_flags, 3 * sizeof(uint32_t));        // pool used, pool overflow, filter passes run
     if (wsrc.total) b1.zero(c->d_pflags, kMaxPrunePasses * sizeof(uint32_t));
     // per pass, the pattern vertices that lost candidates (three arrays in rotation)
     const uint32_t nchg = n * kMaxV;
@@ -734,7 +1294,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             hipLaunchKernelGGL(k_prune, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc, c->d_pflags + passes,
                                passes ? c->d_pflags + passes - 1 : nullptr,
                                passes ? d_chg + (size_t)((passes + 2) % 3) * nchg : nullptr,
-                               d_chg + (size_t)(passes % 3) * nchg, d_chg + (size_t)((passes + 1) % 3) * nchg, nchg);
+                               d_chg + (size_t)(passes % 3) * nchg, d_chg + (size_t)((passes + 1) % 3) * nchg, nchg,
+                               c->d_flags + 2);
         }
         hipLaunchKernelGGL(k_compact, dim3((uint32_t)pair_p.size()), dim3(kBlock), 0, s, c->g, d_ps, d_st, d_pair_p,
                            d_pair_u, passes ? c->d_pflags + passes - 1 : (const uint32_t *)nullptr, d_more);
@@ -874,7 +1435,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         va.st = d_st;
         va.task_base = d_task_base;
         va.P = n;
-        va.total = total_tasks;
+        va.epoch = 0;
         va.ctr = c->d_ctr;
         va.pool = c->d_pool;
         va.aux_region = aux_region;
@@ -924,7 +1485,6 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             uint32_t *keys = c->work.alloc<uint32_t>(total_tasks);
             // counting sort: pg blocks own contiguous task chunks (pg <= kPrioBlocks)
             const uint32_t pg = (uint32_t)std::min<unsigned long long>(kPrioBlocks, (total_tasks + kBlock - 1) / kBlock);
-            const uint32_t chunk = (uint32_t)((total_tasks + pg - 1) / pg);
             uint32_t *hist = c->work.alloc<uint32_t>((size_t)kPrioBins * pg);
             uint32_t *order = c->work.alloc<uint32_t>(total_tasks);
             uint32_t *pu = c->work.alloc<uint32_t>(total_tasks);          // (pattern << 4) | vertex per task
@@ -943,14 +1503,14 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                 // does not expect frequent
                 const uint32_t gs = grid_for((total_tasks + kSample - 1) / kSample, c->num_cus);
                 hipLaunchKernelGGL(k_rank_scores, dim3(gs), dim3(kBlock), 0, s, c->g, d_ps, va.task_base, n,
-                                   total_tasks, c->d_rank_w, logits, ssums, qsums, pu, kSample, (const float *)nullptr);
+                                   c->d_rank_w, logits, ssums, qsums, pu, kSample, (const float *)nullptr);
                 hipLaunchKernelGGL(k_rank_scores, dim3(gt), dim3(kBlock), 0, s, c->g, d_ps, va.task_base, n,
-                                   total_tasks, c->d_rank_w, logits, sums, qsums, pu, 1u, (const float *)ssums);
+                                   c->d_rank_w, logits, sums, qsums, pu, 1u, (const float *)ssums);
             }
             if (cutoff)
                 hipLaunchKernelGGL(k_defer_cutoff, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d_ps, n, sums,
                                    qsums, cutoff);
-            hipLaunchKernelGGL(k_priority, dim3(pg), dim3(kBlock), 0, s, d_ps, va.task_base, n, total_tasks, chunk,
+            hipLaunchKernelGGL(k_priority, dim3(pg), dim3(kBlock), 0, s, d_ps, va.task_base, n,
                                logits, sums, cutoff, keys, pu, hist, feat == nullptr,
                                feat == nullptr ? (const float *)ssums : (const float *)nullptr);
             const uint32_t nhist = kPrioBins * pg, ntiles = (nhist + kScanTile - 1) / kScanTile;
@@ -960,7 +1520,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, hist, nhist, tiles);
             hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, tiles, ntiles);
             hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(kBlock), 0, s, hist, nhist, tiles);
-            hipLaunchKernelGGL(k_prio_scatter, dim3(pg), dim3(kBlock), 0, s, keys, pu, total_tasks, chunk, hist, order,
+            hipLaunchKernelGGL(k_prio_scatter, dim3(pg), dim3(kBlock), 0, s, keys, pu, total_tasks, hist, order,
                                order_pu);
             HIPX(hipGetLastError());
             va.order = order;
@@ -1054,8 +1614,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     }
     // the two launches' counters and the pool words follow each other: one copy
     SearchCounters *hctr = c->h_ctr.ensure(3);
-    HIPX(hipMemcpyAsync(hctr, c->d_ctr, 2 * sizeof(SearchCounters) + 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    const uint32_t *hpool = reinterpret_cast<const uint32_t *>(hctr + 2);   // pool used, pool overflow
+    HIPX(hipMemcpyAsync(hctr, c->d_ctr, 2 * sizeof(SearchCounters) + 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    const uint32_t *hpool = reinterpret_cast<const uint32_t *>(hctr + 2);   // pool used, pool overflow, passes
     sync_spin(c);
     if (hpool[1]) {
         // the tables did not fit: grow the pool to twice the measured need and redo the call
@@ -1068,118 +1628,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         HIPX(hipMalloc(&c->d_pool, (size_t)c->pool_cap * sizeof(uint32_t)));
         return run_eval(c, call, attempt + 1);
     }
-    std::memset(&c->stats, 0, sizeof(c->stats));
-    HIPX(hipEventElapsedTime(&c->stats.filter_ms, c->ev[0], c->ev[1]));
-    HIPX(hipEventElapsedTime(&c->stats.table_ms, c->ev[1], c->ev[2]));
-    if (total_tasks) {
-        HIPX(hipEventElapsedTime(&c->stats.search_ms, c->ev[4], c->ev[5]));
-        c->stats.search_launches = verify_launches;
-        HIPX(hipEventElapsedTime(&c->stats.rank_ms, c->ev[2], c->ev[4]));
-        c->stats.search_splits = std::min(hctr->split_tail, kSplitCap);
-        c->stats.search_watchdog = hctr->watchdog ? 1u : 0u;
-        if (verify_launches == 2) {
-            c->stats.search_splits += std::min(hctr[1].split_tail, kSplitCap);
-            if (hctr[1].watchdog) c->stats.search_watchdog = 1;
-        }
-    }
-    {
-        // algorithmic bytes (definitions in tfsm_gpu.h)
-        // the pool reservation is the neighbour ids the table build scanned
-        unsigned long long tb = 4ull * hpool[0], sb = 0;
-        for (uint32_t p = 0; p < n; ++p) {
-            if (decided[p] != TFSM_DECIDED_SEARCH) continue;
-            const PatStatic &P = hs[p];
-            const PatState &S = hst[p];
-            for (uint32_t sl = 0; sl < P.nslots; ++sl) {
-                const unsigned long long rows = P.dom_size[P.slot_a[sl]];
-                tb += 16ull * rows + 4ull * S.aux_total[sl];
-            }
-            for (uint32_t u = 0; u < P.nv; ++u) {
-                if (!P.dom_size[u]) continue;
-                double per = 0;
-                for (uint32_t sl = 0; sl < P.nslots; ++sl)
-                    if (P.slot_a[sl] == u) per += 8.0 + 4.0 * (double)S.aux_total[sl] / (double)P.dom_size[u];
-                sb += (unsigned long long)(per * (double)(S.valid_cnt[u] + S.invalid_cnt[u]));
-            }
-        }
-        c->stats.table_bytes = tb;
-        c->stats.search_bytes = sb;
-    }
-
-    // ---------------- results and retained child domains ----------------
-    std::vector<CopyDesc> keep;                 // the retained domains' copies, one launch
-    for (uint32_t p = 0; p < n; ++p) {
-        const PatStatic &P = hs[p];
-        const PatState &S = hst[p];
-        tfsm_pattern_result &r = call.out[p];
-        std::memset(&r, 0, sizeof(r));
-        r.decided_by = decided[p];
-        uint32_t lo = 0xffffffffu, hi = 0xffffffffu;
-        for (uint32_t u = 0; u < P.nv; ++u) {
-            r.domain_size[u] = P.dom_size[u];
-            if (decided[p] == TFSM_DECIDED_EXPRESS) {
-                r.valid_count[u] = P.dom_size[u];
-            } else if (decided[p] == TFSM_DECIDED_SEARCH) {
-                r.valid_count[u] = S.valid_cnt[u];
-                r.invalid_count[u] = S.invalid_cnt[u];
-            }
-            lo = std::min(lo, r.valid_count[u]);
-            hi = std::min(hi, r.domain_size[u] - r.invalid_count[u]);
-            r.verdicts += r.valid_count[u] + r.invalid_count[u];
-        }
-        r.support_lower = lo;
-        r.support_upper = hi;
-        r.searches = S.searches;
-        if (fraction) {
-            // a pattern stopped by the valid-count bound (status 2) is
-            // infrequent; every other search- or express-decided one has all
-            // its verdicts, so its sums are exact
-            double mn = 0;
-            for (uint32_t u = 0; u < P.nv; ++u) {
-                r.fraction[u] = frac_value(hfrac[(size_t)p * kMaxV + u]);
-                mn = u == 0 ? r.fraction[u] : std::min(mn, r.fraction[u]);
-            }
-            r.fraction_support = mn;
-            r.frequent = decided[p] != TFSM_DECIDED_FILTER && !(decided[p] == TFSM_DECIDED_SEARCH && S.status == 2) &&
-                         mn >= (double)call.freq;
-        } else if (decided[p] == TFSM_DECIDED_SEARCH && opt.mode == TFSM_MODE_CAPPED && S.status != 0) {
-            r.frequent = S.status == 1;
-        } else {
-            r.frequent = decided[p] != TFSM_DECIDED_FILTER && lo >= call.freq;
-        }
-        if (r.frequent && opt.retain_domains) {
-            size_t words = 0;
-            for (uint32_t u = 0; u < P.nv; ++u) words += P.dom_size[u] + (P.dom_size[u] + 31) / 32;
-            Retained R;
-            R.nv = P.nv;
-            for (uint32_t u = 0; u < P.nv; ++u) R.vlab[u] = P.vlab[u];
-            R.buf = c->cache.get(std::max<size_t>(words, 1) * sizeof(uint32_t), &R.cls);
-            uint32_t *b = static_cast<uint32_t *>(R.buf);
-            for (uint32_t u = 0; u < P.nv; ++u) {
-                const size_t wds = (P.dom_size[u] + 31) / 32;
-                R.size[u] = P.dom_size[u];
-                R.dom[u] = b; b += P.dom_size[u];
-                R.invalid[u] = b; b += wds;
-                if (P.dom_size[u]) keep.push_back({P.dom[u], R.dom[u], P.dom_size[u]});
-                if (wds) keep.push_back({decided[p] == TFSM_DECIDED_SEARCH ? P.invalid_bits[u] : nullptr, R.invalid[u], wds});
-            }
-            r.domain_handle = c->next_handle++;
-            c->retained.emplace(r.domain_handle, R);
-        }
-    }
-    if (!keep.empty()) {
-        const CopyDesc *d_keep = upload(c, keep);
-        hipLaunchKernelGGL(k_copy_batch, dim3((uint32_t)std::min<size_t>(keep.size(), 4096)), dim3(kBlock), 0, s,
-                           d_keep, (uint32_t)keep.size());
-        HIPX(hipGetLastError());
-    }
-    c->last_static = hs;
-    c->last_state.assign(hst, hst + n);
-    c->last_pool_used = hpool[0];
-    c->last_decided = decided;
-    // the retained-domain copies stay queued: the next call waits for the stream before it
-    // reuses the staging, and every function that reads device memory from the host syncs first
-    return TFSM_OK;
+    return finish_call(c, call, Finished{hs, hst, hctr, hpool, decided, hfrac, total_tasks, verify_launches,
+                                         TFSM_LEVEL_HOST, true});
 }
 
 }  // namespace
@@ -1321,6 +1771,15 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         HIPX(hipMalloc(&c->d_rank_w, kParams * sizeof(uint16_t)));
         // the no-allocation gradient's scratch: this device's copy of the array
         HIPX(hipGetSymbolAddress(reinterpret_cast<void **>(&c->d_static_grad), HIP_SYMBOL(scratch)));
+        // the descriptor of device-driven level calls: one slot of this device's c_level
+        HIPX(hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, c->device));
+        if (c->wall_khz <= 0) c->wall_khz = 100000;
+        c->level_slot = acquire_slot(c->device);
+        if (c->level_slot != kNoSlot) {
+            LevelArgs *base = nullptr;
+            HIPX(hipGetSymbolAddress(reinterpret_cast<void **>(&base), HIP_SYMBOL(c_level)));
+            c->d_level = base + c->level_slot;
+        }
         return TFSM_OK;
     });
     if (rc != TFSM_OK) {
@@ -1364,6 +1823,9 @@ int tfsm_ctx_destroy(tfsm_ctx *c) {
     (void)hipFree(c->d_rank_w);
     (void)hipFree(c->d_feat);
     (void)hipFree(c->d_trace);
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
+    c->graphs.clear();
+    release_slot(c->device, c->level_slot);
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->ev_sync) (void)hipEventDestroy(c->ev_sync);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1413,8 +1875,9 @@ int tfsm_eval_level(tfsm_ctx *ctx, const tfsm_pattern *ps, const uint64_t *paren
                 so.num_pushdown = (uint32_t)sp.size();
                 so.pushdown = sp.data();
             }
-            const int rc = run_eval(ctx, EvalCall{ps + i, parent_handles ? parent_handles + i : nullptr, j - i, freq,
-                                                  so, outs + i});
+            EvalCall ec{ps + i, parent_handles ? parent_handles + i : nullptr, j - i, freq, so, outs + i};
+            ec.level_call = true;
+            const int rc = run_eval(ctx, ec);
             if (rc != TFSM_OK) return rc;
             const tfsm_call_stats &s = ctx->stats;
             sum.filter_ms += s.filter_ms;
@@ -1426,6 +1889,8 @@ int tfsm_eval_level(tfsm_ctx *ctx, const tfsm_pattern *ps, const uint64_t *paren
             sum.search_bytes += s.search_bytes;
             sum.search_splits += s.search_splits;
             sum.search_watchdog |= s.search_watchdog;
+            sum.filter_passes += s.filter_passes;
+            sum.level_mode = s.level_mode;
             ++slices;
             i = j;
         }

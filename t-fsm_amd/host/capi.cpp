@@ -143,11 +143,12 @@ int64_t tfsmh_mine(const tfsmh_graph *h, tfsm_ctx *ctx, int freq, int max_nodes,
         }
     }
     if (stats_out) {
-        double v[16] = {st.mining_s, (double)st.evaluated, (double)st.frequent, (double)st.verdicts, (double)st.searches,
+        double v[20] = {st.mining_s, (double)st.evaluated, (double)st.frequent, (double)st.verdicts, (double)st.searches,
                         (double)st.eval_calls, st.gpu_ms[0], st.gpu_ms[1], st.gpu_ms[2], (double)st.search_launches,
                         (double)st.bytes_table, (double)st.bytes_search, (double)st.levels.size(),
                         st.levels.empty() ? 0.0 : (double)st.levels.back().size, (double)st.domain_sum,
-                        (double)st.pushdown_sources};
+                        (double)st.pushdown_sources, (double)st.filter_passes, (double)st.device_calls,
+                        (double)st.graph_calls, 0.0};
         std::memcpy(stats_out, v, sizeof(v));
     }
     return (int64_t)found.size();

@@ -191,6 +191,9 @@ int mine(const DataGraph &g, tfsm_ctx *ctx, const MinerOptions &opt, std::vector
             stats.search_kernel_ms += cs.search_ms;
             stats.bytes_table += cs.table_bytes;
             stats.bytes_search += cs.search_bytes;
+            stats.filter_passes += cs.filter_passes;
+            stats.device_calls += cs.level_mode == TFSM_LEVEL_DEVICE || cs.level_mode == TFSM_LEVEL_GRAPH;
+            stats.graph_calls += cs.level_mode == TFSM_LEVEL_GRAPH;
         };
         if (opt.batched) {
             std::vector<tfsm_pattern> sub;

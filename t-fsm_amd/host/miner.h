@@ -68,6 +68,9 @@ struct MinerStats {
     uint64_t domain_sum = 0;        // filtered domain sizes summed over the evaluated patterns
     double eval_wall_ms = 0;        // host wall time inside the eval calls
     double growth_wait_ms = 0;      // host wall time waiting for the next level's growth
+    uint64_t filter_passes = 0;     // DP-iso passes that did work
+    uint64_t device_calls = 0;      // eval calls run device-driven (TFSM_LEVEL_DEVICE or _GRAPH)
+    uint64_t graph_calls = 0;       // of those, launched as one captured hipGraph
 };
 
 // Runs the whole mining job; `found` receives the frequent patterns in a

@@ -20,6 +20,10 @@ ORDER_REFERENCE = 0
 ORDER_LEARNED = 1
 MATCH_ADAPTIVE = 0
 MATCH_REFERENCE = 1
+LEVEL_DEFAULT = 0
+LEVEL_HOST = 1
+LEVEL_DEVICE = 2
+LEVEL_GRAPH = 3
 DECIDED_FILTER, DECIDED_EXPRESS, DECIDED_SEARCH = 0, 1, 2
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "no usable GPU",
@@ -54,7 +58,7 @@ class EvalOptions(ctypes.Structure):
                 ("group_lanes", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
                 ("num_pushdown", ctypes.c_uint32), ("pushdown", ctypes.c_void_p),
                 ("steps_per_round", ctypes.c_uint32), ("refill_min", ctypes.c_uint32),
-                ("refill_max", ctypes.c_uint32), ("prune_batch", ctypes.c_uint32)]
+                ("refill_max", ctypes.c_uint32), ("prune_batch", ctypes.c_uint32), ("level_mode", ctypes.c_uint32)]
 
 
 class PatternResult(ctypes.Structure):
@@ -70,7 +74,8 @@ class CallStats(ctypes.Structure):
     _fields_ = [("filter_ms", ctypes.c_float), ("table_ms", ctypes.c_float), ("search_ms", ctypes.c_float),
                 ("search_launches", ctypes.c_uint32), ("table_bytes", ctypes.c_uint64),
                 ("search_bytes", ctypes.c_uint64), ("search_splits", ctypes.c_uint32),
-                ("search_watchdog", ctypes.c_uint32), ("rank_ms", ctypes.c_float)]
+                ("search_watchdog", ctypes.c_uint32), ("rank_ms", ctypes.c_float), ("level_mode", ctypes.c_uint32),
+                ("filter_passes", ctypes.c_uint32)]
 
 
 RANKER_FEATURES = 24
@@ -429,7 +434,7 @@ class Context:
              results_path=None, keys_path=None, pushdown=False, **tuning):
         """tuning: search-kernel shape for every evaluation (see eval_level);
         pushdown: push-down pruning from cached frequent sub-patterns."""
-        stats = (ctypes.c_double * 16)()
+        stats = (ctypes.c_double * 20)()
         opt = EvalOptions(**tuning)
         n = host_lib().tfsmh_mine(self.graph.h, self.h, int(freq), int(max_nodes), int(threads), int(batched),
                                   int(mode), int(learned), int(pushdown), results_path.encode() if results_path else None,
@@ -438,7 +443,7 @@ class Context:
             raise TfsmError(int(n), host_lib().tfsmh_last_error().decode())
         names = ["mining_s", "evaluated", "frequent", "verdicts", "searches", "eval_calls", "filter_ms", "table_ms",
                  "search_ms", "search_launches", "bytes_table", "bytes_search", "levels", "max_size", "domain_sum",
-                 "pushdown_sources"]
+                 "pushdown_sources", "filter_passes", "device_calls", "graph_calls"]
         return int(n), dict(zip(names, list(stats)[:len(names)]))
 
     def close(self):
@@ -461,7 +466,7 @@ def load_tuning(path):
         raise TfsmError(-1, host_lib().tfsmh_last_error().decode())
     names = ["grid_waves_per_cu", "split_steps", "split_fanout", "long_steps", "long_waves_per_cu",
              "long_donate_every", "long_fanout", "match_order", "group_lanes", "steps_per_round", "refill_min",
-             "refill_max", "prune_batch"]
+             "refill_max", "prune_batch", "level_mode"]
     return {k: getattr(opt, k) for k in names if getattr(opt, k)}
 
 
