@@ -114,8 +114,13 @@ TFSM_LANE_FN uint32_t lane_gallop(const A &a, uint32_t lo, uint32_t hi, uint32_t
 // caller serves; uniform on the device).
 // C: idx(x), vid(j), pos(d), end(d), cur(g), cend(g), curv(g), meta(d), all
 // lvalues.
-template <int MV, int ORD = kOrderMixed>
+// ME: the most pattern edges this instance's adaptive order handles (a
+// multiple of 4 up to kCutMax, the edges packed four to a word); a launch
+// whose patterns have at most 8 edges runs the ME = 8 instance, whose
+// per-edge arrays take half the registers.
+template <int MV, int ORD = kOrderMixed, int ME = (int)kCutMax>
 struct Lane {
+    static_assert(ME % 4 == 0 && ME > 0 && ME <= (int)kCutMax, "edges packed four to a word");
     uint32_t u = 0, n = 0, d = 0, floor = 1;
     // interleaved root search: of depth `floor`'s candidates this lane tests
     // every stride-th one (the other lanes of its group take the others)
@@ -128,17 +133,17 @@ struct Lane {
     // search kernel's occupancy is set by its LDS, so registers are spare):
     // the packed edges, where each slot's row pairs start, and for patterns
     // of up to 8 vertices the same-label masks (8 bits per vertex).
-    uint32_t ep[kCutMax / 4] = {};
-    uint32_t aor[2 * kCutMax] = {};
+    uint32_t ep[ME / 4] = {};
+    uint32_t aor[2 * ME] = {};
     unsigned long long slab = 0;     // a value, not an array: a run-time index would put it in scratch
 
     template <class A>
     TFSM_LANE_FN void load_edges(const A &a) {
 #pragma unroll
-        for (uint32_t i = 0; i < kCutMax / 4; ++i) ep[i] = a.edge_pack(i);
+        for (uint32_t i = 0; i < (uint32_t)ME / 4; ++i) ep[i] = a.edge_pack(i);
         const uint32_t me = a.edge_max();
 #pragma unroll
-        for (uint32_t e = 0; e < kCutMax; ++e) {
+        for (uint32_t e = 0; e < (uint32_t)ME; ++e) {
             if (e >= me) continue;
             // slots of absent edges are never selected; slot 0 always exists
             const bool edge = ((ep[e >> 2] >> ((e & 3u) * 8u)) & 0xffu) != 0xffu;
@@ -211,10 +216,10 @@ struct Lane {
         // me: the most edges of any pattern of the launch (uniform), so the
         // wave skips the slots no pattern has
         const uint32_t me = a.edge_max();
-        uint32_t sel[kCutMax], rr[kCutMax];
+        uint32_t sel[ME], rr[ME];
         uint32_t cut = 0, dlo = 0, dhi = 0;     // cut edges; their unmatched ends, 4 bits each
 #pragma unroll
-        for (uint32_t e = 0; e < kCutMax; ++e) {
+        for (uint32_t e = 0; e < (uint32_t)ME; ++e) {
             if (e >= me) continue;
             const uint32_t xy = (ep[e >> 2] >> ((e & 3u) * 8u)) & 0xffu;
             const bool edge = xy != 0xffu;
@@ -229,15 +234,15 @@ struct Lane {
             if (e < 8) dlo |= (is_cut ? dst : 0u) << (4u * e);
             else dhi |= (is_cut ? dst : 0u) << (4u * (e - 8));
         }
-        uint32_t ao[kCutMax];
+        uint32_t ao[ME];
 #pragma unroll
-        for (uint32_t e = 0; e < kCutMax; ++e) {
+        for (uint32_t e = 0; e < (uint32_t)ME; ++e) {
             if (e >= me) continue;
             ao[e] = (sel[e] & 1u) ? aor[2 * e + 1] : (sel[e] ? aor[2 * e] : aor[0]);
         }
-        uint32_t lo[kCutMax], hi[kCutMax];
+        uint32_t lo[ME], hi[ME];
 #pragma unroll
-        for (uint32_t e = 0; e < kCutMax; ++e) {
+        for (uint32_t e = 0; e < (uint32_t)ME; ++e) {
             if (e >= me) continue;
             a.row_at(ao[e], rr[e], lo[e], hi[e]);
         }
@@ -246,7 +251,7 @@ struct Lane {
         if (w == kNone) {
             uint32_t best = 0xffffffffu;
 #pragma unroll
-            for (uint32_t e = 0; e < kCutMax; ++e) {
+            for (uint32_t e = 0; e < (uint32_t)ME; ++e) {
                 if (e >= me) continue;
                 const uint32_t l = len_of(e);
                 if (((cut >> e) & 1u) && l < best) {
@@ -257,7 +262,7 @@ struct Lane {
         }
         uint32_t nb = 0, pick = 0, plen = 0xffffffffu;
 #pragma unroll
-        for (uint32_t e = 0; e < kCutMax; ++e) {
+        for (uint32_t e = 0; e < (uint32_t)ME; ++e) {
             if (e >= me) continue;
             if (((cut >> e) & 1u) && dst_of(e) == w) {
                 if (len_of(e) < plen) {
@@ -272,7 +277,7 @@ struct Lane {
         const uint32_t secb = dd ? meta_secb(pm) + meta_secn(pm) : 0u;
         uint32_t k = 0, g = secb, plo = 0, phi = 0;
 #pragma unroll
-        for (uint32_t e = 0; e < kCutMax; ++e) {
+        for (uint32_t e = 0; e < (uint32_t)ME; ++e) {
             if (e >= me) continue;
             if (((cut >> e) & 1u) && dst_of(e) == w) {
                 if (k == prim) {

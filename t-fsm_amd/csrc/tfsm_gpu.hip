@@ -228,7 +228,7 @@ constexpr size_t kDeviceRegionBudget = size_t(4) << 30;
 constexpr uint32_t kNoSlot = 0xffffffffu;
 // launch grids of a device-driven call, which with the options make a graph's key
 struct LevelShape {
-    uint32_t learned, passes, gfill, gsrc, gpair, gaux, gsample, grank, vgrid;
+    uint32_t learned, passes, gfill, gsrc, gpair, gaux, gsample, grank, vgrid, me8;
     bool operator==(const LevelShape &o) const { return std::memcmp(this, &o, sizeof(o)) == 0; }
 };
 
@@ -765,9 +765,14 @@ void launch_level(tfsm_ctx *c, const LevelShape &sh, bool events) {
         hipLaunchKernelGGL(k_prio_scatter_g, dim3(kGraphPrioBlocks), dim3(kBlock), 0, s, slot);
     }
     if (events) HIPX(hipEventRecord(c->ev[4], s));
-    hipLaunchKernelGGL((k_verify_g<8, tfsm_lane::kOrderAdaptive>), dim3(sh.vgrid), dim3(kBlock), 0, s, slot, 0u);
-    if (sh.learned)
-        hipLaunchKernelGGL((k_verify_g<8, tfsm_lane::kOrderAdaptive>), dim3(sh.vgrid), dim3(kBlock), 0, s, slot, 1u);
+    for (uint32_t part = 0; part < (sh.learned ? 2u : 1u); ++part) {
+        if (sh.me8)
+            hipLaunchKernelGGL((k_verify_g<8, tfsm_lane::kOrderAdaptive, 8>), dim3(sh.vgrid), dim3(kBlock), 0, s, slot,
+                               part);
+        else
+            hipLaunchKernelGGL((k_verify_g<8, tfsm_lane::kOrderAdaptive, 12>), dim3(sh.vgrid), dim3(kBlock), 0, s,
+                               slot, part);
+    }
     if (events) {
         HIPX(hipEventRecord(c->ev[5], s));
         HIPX(hipEventRecord(c->ev[3], s));
@@ -955,6 +960,7 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     const uint32_t per_cu = opt.grid_waves_per_cu
                                 ? std::min((uint32_t)c->verify_blocks8a, std::max(1u, opt.grid_waves_per_cu / (kBlock / 64)))
                                 : (uint32_t)c->verify_blocks8a;
+    // (verify_blocks8a: the fewer resident blocks of the ME = 8 and ME = 12 instances)
     for (uint32_t part = 0; part < 2; ++part) {
         VerifyArgs &va = L->va[part];
         va.ps = L->ps;
@@ -1009,6 +1015,7 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     sh.gsample = grid_class((task_ub + kSample - 1) / kSample, (uint32_t)c->num_cus * 8);
     sh.grank = grid_class(task_ub, (uint32_t)c->num_cus * 8);
     sh.vgrid = (uint32_t)c->num_cus * per_cu;
+    sh.me8 = max_edges <= 8;
     // the batch's uploads, then the descriptor, then the sequence
     b.flush();
     HIPX(hipMemcpyAsync(c->d_level, L, sizeof(LevelArgs), hipMemcpyHostToDevice, s));
@@ -1572,11 +1579,17 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         const bool two_parts = va.order && va.split && !va.long_cap;
         auto launch_verify = [&]() {
             if (max_nv > 8)
-                hipLaunchKernelGGL((k_verify<16, tfsm_lane::kOrderMixed>), dim3(grid), dim3(kBlock), 0, s, va, epoch);
+                hipLaunchKernelGGL((k_verify<16, tfsm_lane::kOrderMixed, 12>), dim3(grid), dim3(kBlock), 0, s, va,
+                                   epoch);
+            else if (all_adaptive && va.max_edges <= 8)
+                hipLaunchKernelGGL((k_verify<8, tfsm_lane::kOrderAdaptive, 8>), dim3(grid), dim3(kBlock), 0, s, va,
+                                   epoch);
             else if (all_adaptive)
-                hipLaunchKernelGGL((k_verify<8, tfsm_lane::kOrderAdaptive>), dim3(grid), dim3(kBlock), 0, s, va, epoch);
+                hipLaunchKernelGGL((k_verify<8, tfsm_lane::kOrderAdaptive, 12>), dim3(grid), dim3(kBlock), 0, s, va,
+                                   epoch);
             else
-                hipLaunchKernelGGL((k_verify<8, tfsm_lane::kOrderMixed>), dim3(grid), dim3(kBlock), 0, s, va, epoch);
+                hipLaunchKernelGGL((k_verify<8, tfsm_lane::kOrderMixed, 12>), dim3(grid), dim3(kBlock), 0, s, va,
+                                   epoch);
             HIPX(hipGetLastError());
         };
         va.part = two_parts ? 1u : 0u;
@@ -1671,11 +1684,14 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         c->num_cus = prop.multiProcessorCount;
         // the persistent search grid is sized to what stays resident
         // (registers and LDS of each instance)
-        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->verify_blocks8, k_verify<8, tfsm_lane::kOrderMixed>,
+        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->verify_blocks8, k_verify<8, tfsm_lane::kOrderMixed, 12>,
                                                           kBlock, 0));
-        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->verify_blocks8a, k_verify<8, tfsm_lane::kOrderAdaptive>,
-                                                          kBlock, 0));
-        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->verify_blocks16, k_verify<16, tfsm_lane::kOrderMixed>,
+        int b8 = 1;
+        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->verify_blocks8a,
+                                                          k_verify<8, tfsm_lane::kOrderAdaptive, 12>, kBlock, 0));
+        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b8, k_verify<8, tfsm_lane::kOrderAdaptive, 8>, kBlock, 0));
+        c->verify_blocks8a = std::min(c->verify_blocks8a, b8);
+        HIPX(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c->verify_blocks16, k_verify<16, tfsm_lane::kOrderMixed, 12>,
                                                           kBlock, 0));
         c->verify_blocks8 = std::max(1, c->verify_blocks8);
         c->verify_blocks8a = std::max(1, c->verify_blocks8a);

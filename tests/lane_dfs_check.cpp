@@ -54,6 +54,11 @@ namespace {
 }
 
 constexpr int kMV = 8;
+// the search instance's edge bound (the kernel's ME): 12, or 8 with -DLANE_ME=8
+#ifndef LANE_ME
+#define LANE_ME 12
+#endif
+constexpr int kLaneME = LANE_ME;
 
 struct HostCols {
     uint32_t idx_[kMV], vid_[kMV], pos_[kMV], end_[kMV], cur_[kMV], cend_[kMV], curv_[kMV], meta_[kMV];
@@ -186,7 +191,7 @@ struct HostAcc {
     }
     // the library's rule for the adaptive order
     bool adaptive_fits() const {
-        return edges.size() <= tfsm_lane::kCutMax && edges.size() - (n - 1) <= (size_t)std::min(kMV, 15);
+        return edges.size() <= (size_t)kLaneME && edges.size() - (n - 1) <= (size_t)std::min(kMV, 15);
     }
 };
 
@@ -215,7 +220,7 @@ TaskResult run_task_in(const HostAcc &a, uint32_t u, uint32_t ci, uint32_t donat
     TaskResult res;
     HostCols c;
     c.n = a.n;
-    tfsm_lane::Lane<kMV, ORD> L;
+    tfsm_lane::Lane<kMV, ORD, kLaneME> L;
     std::vector<Piece> queue;
     auto check_embedding = [&]() {
         if (c.idx(u) != ci) violation("embedding root", c.idx(u), ci);
@@ -556,7 +561,7 @@ int level_mode(const char *spec_path, uint32_t split_steps, uint32_t fanout, uin
         bool active = false;
         uint32_t task = 0, steps = 0, since = 0;
         HostCols c;
-        tfsm_lane::Lane<kMV, tfsm_lane::kOrderMixed> L;
+        tfsm_lane::Lane<kMV, tfsm_lane::kOrderMixed, kLaneME> L;
     };
     std::vector<VLane> lanes(nlanes);
     std::vector<LevelRecord> fifo;          // reserved records; head never passes the tail

@@ -21,16 +21,27 @@ from test_wave_dfs import variants
 HARNESS_SRC = os.path.join(REPO, "tests", "lane_dfs_check.cpp")
 
 
-@pytest.fixture(scope="module")
-def harness(tmp_path_factory):
+def build_harness(tmp_path_factory, me):
     lib = os.path.join(ORACLE, "_build", "liboracle.so")
     if not os.path.exists(lib):
         subprocess.run(["make", "-C", ORACLE], check=True, stdout=subprocess.DEVNULL)
-    out = str(tmp_path_factory.mktemp("lane") / "lane_dfs_check")
-    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-Wno-unknown-pragmas", "-fsanitize=address,undefined",
-                    "-fno-sanitize-recover=all", "-o", out, HARNESS_SRC, lib,
+    out = str(tmp_path_factory.mktemp("lane") / f"lane_dfs_check_{me}")
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-Wno-unknown-pragmas", "-fsanitize=address,undefined,bounds",
+                    "-fno-sanitize-recover=all", f"-DLANE_ME={me}", "-o", out, HARNESS_SRC, lib,
                     f"-Wl,-rpath,{os.path.dirname(lib)}"], check=True)
     return out
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    return build_harness(tmp_path_factory, 12)
+
+
+@pytest.fixture(scope="module")
+def harness8(tmp_path_factory):
+    """The search instance for launches whose patterns have at most 8 edges
+    (per-edge arrays of 8; patterns with more run in the reference order)."""
+    return build_harness(tmp_path_factory, 8)
 
 
 DONATION = [(0, 1), (1, 1), (2, 3)]     # (donate every N steps (0: never), pieces per donation)
@@ -67,27 +78,28 @@ def test_gnm(harness, golden, freq, donate, workdir, tmp_path):
     assert (out["pieces"] > 0) == (donate[0] > 0), out
 
 
+@pytest.mark.parametrize("me", [12, 8])
 @pytest.mark.parametrize("donate", [(0, 1), (16, 2)])
-def test_powerlaw_sampled(harness, workdir, tmp_path, donate):
+def test_powerlaw_sampled(harness, harness8, me, workdir, tmp_path, donate):
     """Four- and five-vertex patterns of the power-law graph, 40 sampled
     candidates per vertex; the adaptive order takes far fewer steps."""
     path = workload_path("powerlaw100k", 2, workdir)
     pats = [p for p in variants(golden_keys("powerlaw100k_f9000_m5"), 60) if len(p[0]) >= 4][:14]
-    out = run(harness, path, 9000, 40, pats, tmp_path, donate)
+    out = run(harness if me == 12 else harness8, path, 9000, 40, pats, tmp_path, donate)
     assert out["steps_adaptive"] < out["steps_reference"], out
     assert (out["pieces"] > 0) == (donate[0] > 0), out
 
 
-@pytest.mark.parametrize("seed,instances,donate", [(1, 30, (0, 1)), (2, 30, (0, 1)), (3, 6, (16, 4)),
-                                                   (4, 4, (4, 2))])
-def test_random(harness, seed, instances, donate):
+@pytest.mark.parametrize("seed,instances,donate,me", [(1, 30, (0, 1), 12), (2, 30, (0, 1), 12), (3, 6, (16, 4), 12),
+                                                      (4, 4, (4, 2), 12), (6, 30, (0, 1), 8), (7, 6, (16, 4), 8)])
+def test_random(harness, harness8, seed, instances, donate, me):
     """Synthetic tables with empty, short and long rows, against a plain
     recursive search written in the harness: the same verdicts in both orders,
     and in the reference order without pieces the same candidates entered in
     the same order (so a cursor that skips or repeats a table entry shows
     up)."""
-    r = subprocess.run([harness, "--random", str(seed), str(instances), str(donate[0]), str(donate[1])],
-                       capture_output=True, text=True, timeout=600)
+    r = subprocess.run([harness if me == 12 else harness8, "--random", str(seed), str(instances), str(donate[0]),
+                        str(donate[1])], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     f = r.stdout.split()
     out = dict(zip(f[1::2], (int(x) for x in f[2::2])))
@@ -120,8 +132,8 @@ def test_random_lane_groups(harness, group):
     assert out["checked"] > 1000 and out["mismatches"] == 0 and 0 < out["found"] < out["checked"], r.stdout
 
 
-@pytest.mark.parametrize("reference_order", [0, 1], ids=["adaptive", "reference"])
-def test_level_record_queue(harness, reference_order, workdir, tmp_path):
+@pytest.mark.parametrize("reference_order,me", [(0, 12), (1, 12), (0, 8)], ids=["adaptive", "reference", "adaptive-me8"])
+def test_level_record_queue(harness, harness8, reference_order, me, workdir, tmp_path):
     """Several power-law patterns in one launch, as the GPU parity test of a
     whole level with split_steps=64 and four records per donation runs them:
     256 virtual lanes take root tasks of every pattern, then records from one
@@ -137,8 +149,8 @@ def test_level_record_queue(harness, reference_order, workdir, tmp_path):
         f.write(f"{path} 9000 0\n")
         for vl, es in pats:
             f.write(" ".join(str(x) for x in [len(vl)] + list(vl) + [len(es)] + [y for e in es for y in e]) + "\n")
-    r = subprocess.run([harness, "--level", spec, "64", "4", "256", "1", str(reference_order)], capture_output=True,
-                       text=True, timeout=900)
+    r = subprocess.run([harness if me == 12 else harness8, "--level", spec, "64", "4", "256", "1",
+                        str(reference_order)], capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stdout + r.stderr
     f = r.stdout.split()
     out = dict(zip(f[0::2], (int(x) for x in f[1::2])))
