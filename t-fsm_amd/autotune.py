@@ -1,10 +1,10 @@
 """Autotuner for the search kernel's shape on one workload.
 
 Times whole mining runs (graph resident in HBM) for candidate shapes and keeps
-the fastest.  The knobs are the resident waves per CU, the step budget of the
-record splitting, the lanes of a lane group, the DFS steps between bookkeeping
-rounds, the sizes of a wave's root-task refills and the DP-iso passes
-launched between convergence checks.  The space is searched one knob at a
+the fastest.  The knobs are the level driver, the resident waves per CU, the step budget
+of the record splitting, the lanes of a lane group, the DFS steps between
+bookkeeping rounds, the sizes of a wave's root-task refills and the DP-iso
+passes launched between convergence checks.  The space is searched one knob at a
 time from the library's defaults (coordinate descent, two sweeps), keeping
 each knob's best value before the next.  A shape is accepted only if its run
 returns exactly the pattern set of the first (default) shape: the search's
@@ -34,8 +34,11 @@ NEVER = 0xffffffff
 # resident); record splitting after 64 to 1024 steps, or none; lane groups of
 # up to 1-16 lanes; 2-16 steps per bookkeeping round; refills of 16-128 (at
 # least) and 128-1024 (at most) lanes' worth of tasks; 2-9 filter passes per
-# convergence check.
+# convergence check (a device-driven call runs that many passes); and the
+# level driver: device-driven as a replayed hipGraph (0, the default),
+# host-driven (1) or device-driven kernel by kernel (2).
 SPACE = {
+    "level_mode": [0, 1, 2],
     "grid_waves_per_cu": [4, 8],
     "split_steps": [128, 64, 256, 1024, NEVER],
     "group_lanes": [8, 1, 2, 4, 16],
@@ -61,7 +64,14 @@ def main():
     ap.add_argument("--ranker", help="time the learned order (the ranker file to install)")
     ap.add_argument("--repeats", type=int, default=5)
     ap.add_argument("--out", required=True)
+    ap.add_argument("--fix", action="append", default=[], metavar="KNOB=VALUE",
+                    help="hold a knob at one value (e.g. level_mode=0: every level as one replayed hipGraph)")
     args = ap.parse_args()
+    for f in args.fix:
+        knob, value = f.split("=")
+        if knob not in SPACE:
+            ap.error(f"unknown knob {knob}")
+        SPACE[knob] = [int(value)]
 
     tmp = tempfile.mkdtemp(prefix="tfsm_tune_")
     lg = args.lg
