@@ -129,6 +129,9 @@ struct Lane {
     bool adaptive_rt = false;      // the pattern's order (kOrderMixed)
     TFSM_LANE_FN bool adaptive() const { return ORD == kOrderAdaptive || (ORD == kOrderMixed && adaptive_rt); }
     uint32_t pf_pos = kUnknown, pf_val = 0;   // prefetched pool entry (the pool is read-only)
+    // the depth and candidate the last step entered (kNone: it entered none);
+    // read by the host check only (dead on the device)
+    uint32_t last_depth = kNone, last_cand = 0;
     // Adaptive order: the pattern's constants, loaded once per task (the
     // search kernel's occupancy is set by its LDS, so registers are spare):
     // the packed edges, where each slot's row pairs start, and for patterns
@@ -384,11 +387,16 @@ struct Lane {
     // backtrack one depth.
     template <class C, class A>
     TFSM_LANE_FN uint32_t step(C &c, const A &a) {
-        const uint32_t pos = c.pos(d), end = c.end(d);
-        if (pos >= end) {
+        last_depth = kNone;
+        uint32_t pos = c.pos(d), end = c.end(d);
+        // backtrack out of every exhausted depth, then test the next
+        // candidate in the same step
+        while (pos >= end) {
             if (adaptive()) vis &= ~(1u << meta_vertex(c.meta(d)));
             --d;
-            return d < floor ? kLaneExhausted : kLaneRunning;
+            if (d < floor) return kLaneExhausted;
+            pos = c.pos(d);
+            end = c.end(d);
         }
         const uint32_t adv = d == floor ? stride : 1u;
         c.pos(d) = pos + adv;
@@ -421,13 +429,37 @@ struct Lane {
             // the adaptive order's rows always have cursor slots
             if (ORD == kOrderAdaptive || g < MV) {
                 const uint32_t cur = c.cur(g), hi = c.cend(g);
-                if (cur >= hi) return kLaneRunning;
+                // a spent row: no later candidate of this depth is in it
+                if (cur >= hi) {
+                    c.pos(d) = end;
+                    return kLaneRunning;
+                }
                 uint32_t cv = c.curv(g), at = cur;
                 if (cv == kUnknown) at = lane_gallop(a, cur, hi, cand, cv);
                 else if (cv < cand) at = lane_gallop(a, cur + 1, hi, cand, cv);
                 c.cur(g) = at;
                 c.curv(g) = cv;
                 member = at < hi && cv == cand;
+                if (!member) {
+                    if (at >= hi) {
+                        c.pos(d) = end;
+                        return kLaneRunning;
+                    }
+                    // leapfrog: no primary entry below cv is in this row
+                    if (adv == 1) {
+                        const uint32_t nx = pos + 1;
+                        if (nx < end && !(pf_pos == nx && pf_val >= cv)) {
+                            uint32_t v2;
+                            const uint32_t np = lane_gallop(a, nx, end, cv, v2);
+                            c.pos(d) = np;
+                            if (np < end) {
+                                pf_pos = np;
+                                pf_val = v2;
+                            }
+                        }
+                    }
+                    return kLaneRunning;
+                }
             } else {
                 // reference order only: more non-primary rows than cursor slots
                 uint32_t lo, hi, cv;
@@ -450,6 +482,8 @@ struct Lane {
             }
         }
         c.idx(w) = cand;
+        last_depth = d;
+        last_cand = cand;
         if (d + 1 == n) return kLaneFound;
         ++d;
         if (adaptive()) enter_adaptive(c, a, d, kNone, kNone);

@@ -237,13 +237,10 @@ TaskResult run_task_in(const HostAcc &a, uint32_t u, uint32_t ci, uint32_t donat
     auto run = [&](bool root_piece) {
         uint32_t since = 0;
         while (true) {
-            // the candidate the next step enters, for the visit sequence
-            const uint32_t d0 = L.d;
-            const bool will_test = c.pos(d0) < c.end(d0);
-            const uint32_t cand = will_test ? a.pool(c.pos(d0)) : 0;
             const uint32_t r = L.step(c, a);
             ++res.steps;
-            if (seq && root_piece && will_test && (L.d > d0 || r == tfsm_lane::kLaneFound)) seq->emplace_back(d0, cand);
+            // the candidate the step entered, for the visit sequence
+            if (seq && root_piece && L.last_depth != tfsm_lane::kNone) seq->emplace_back(L.last_depth, L.last_cand);
             if (r == tfsm_lane::kLaneFound) {
                 check_embedding();
                 return r;
