@@ -270,6 +270,8 @@ struct VerifyArgs {
                                            // form the hot tier, searched with the largest groups
 };
 
+constexpr uint32_t kClockWays = 16;
+
 // Everything the kernels of one device-driven level call read, in one
 // descriptor per context slot (c_level in kernels.inc).  The kernels of such
 // a call take only the slot (and a pass or part number), so their launch
@@ -315,11 +317,13 @@ struct LevelArgs {
     uint32_t nhist, ntiles;
     // existence search: the first launch and the deferred one
     VerifyArgs va[2];
-    // phase clock (the device's constant-rate wall clock): [0] the first
-    // block's start (min), [1] the end of the filter and the plan, [2] of the
-    // tables (max),
-    // [3] the first search block's start (min), [4] the last search wave's end
-    // (max); the host uploads the starting values with the call
+    // phase clock (the device's constant-rate wall clock), five stamps of
+    // kClockWays words each, a block using word (block mod kClockWays) so
+    // that the atomics spread over lines: [0] the first block's start (min),
+    // [1] the end of the filter and the plan, [2] of the tables (max), [3]
+    // the first search block's start (min), [4] the last search block's end
+    // (max); the host uploads the starting values with the call and reduces
+    // each stamp's words
     unsigned long long *clk;
 };
 

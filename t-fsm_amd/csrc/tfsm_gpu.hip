@@ -918,7 +918,8 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     L->fills = b.add(b.zeros);
     b.zeros.clear();
     // the phase clock's starting values (min slots at the largest value)
-    const std::vector<unsigned long long> clk0{~0ull, 0ull, 0ull, ~0ull, 0ull};
+    std::vector<unsigned long long> clk0(5 * kClockWays, 0ull);
+    for (uint32_t w = 0; w < kClockWays; ++w) clk0[w] = clk0[3 * kClockWays + w] = ~0ull;
     L->clk = b.add(clk0);
     L->P = n;
     L->g = c->g;
@@ -1032,8 +1033,9 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     SearchCounters *hctr = c->h_ctr.ensure(3);
     HIPX(hipMemcpyAsync(hctr, c->d_ctr, 2 * sizeof(SearchCounters) + 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     const uint32_t *hpool = reinterpret_cast<const uint32_t *>(hctr + 2);
-    unsigned long long *hclk = c->h_clk.ensure(5);
-    if (graph) HIPX(hipMemcpyAsync(hclk, L->clk, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    unsigned long long *hclk = c->h_clk.ensure(5 * kClockWays);
+    if (graph)
+        HIPX(hipMemcpyAsync(hclk, L->clk, 5 * kClockWays * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     sync_spin(c);
     if (*reinterpret_cast<const uint32_t *>(hst + n)) return kRedoHost;
     if (hpool[1]) {
@@ -1064,7 +1066,17 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     if (graph) {
         // the phase clock in ms: first block .. compaction .. tables .. search start .. end
         const double per_ms = (double)c->wall_khz;
-        auto span = [&](int a, int b) { return hclk[b] > hclk[a] ? (float)((double)(hclk[b] - hclk[a]) / per_ms) : 0.f; };
+        // each stamp: the minimum (starts) or maximum (ends) over its words
+        unsigned long long st[5];
+        for (uint32_t k = 0; k < 5; ++k) {
+            const bool start = k == 0 || k == 3;
+            st[k] = start ? ~0ull : 0ull;
+            for (uint32_t w = 0; w < kClockWays; ++w) {
+                const unsigned long long x = hclk[k * kClockWays + w];
+                st[k] = start ? std::min(st[k], x) : std::max(st[k], x);
+            }
+        }
+        auto span = [&](int a, int b) { return st[b] > st[a] ? (float)((double)(st[b] - st[a]) / per_ms) : 0.f; };
         c->stats.filter_ms = span(0, 1);
         c->stats.table_ms = span(1, 2);
         if (total_tasks) {
