@@ -148,6 +148,9 @@ typedef struct {
                                    and is redone host-driven when they do not reach the fixpoint */
     uint32_t level_mode;        /* TFSM_LEVEL_* (0: device-driven with a captured hipGraph for
                                    tfsm_eval_level, host-driven for tfsm_eval_pattern) */
+    uint32_t hot_group_lanes;   /* most lanes per task of the learned order's hot tier (the deciding
+                                   vertex's tasks of the patterns expected infrequent; 0: group_lanes,
+                                   else a power of two <= 64); verdicts do not depend on it */
 } tfsm_eval_options;
 
 /* How one call's launch sequence is driven.  None changes a result.

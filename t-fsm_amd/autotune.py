@@ -2,7 +2,7 @@
 
 Times whole mining runs (graph resident in HBM) for candidate shapes and keeps
 the fastest.  The knobs are the level driver, the resident waves per CU, the step budget
-of the record splitting, the lanes of a lane group, the DFS steps between
+of the record splitting, the lanes of a lane group (and of the hot tier's), the DFS steps between
 bookkeeping rounds, the sizes of a wave's root-task refills and the DP-iso
 passes launched between convergence checks.  The space is searched one knob at a
 time from the library's defaults (coordinate descent, two sweeps), keeping
@@ -32,7 +32,8 @@ NEVER = 0xffffffff
 # Values the tuner may choose from, the library's default first: workgroups of
 # 4 waves, 1 or 2 resident per CU (the library clamps to what stays
 # resident); record splitting after 64 to 1024 steps, or none; lane groups of
-# up to 1-16 lanes; 2-16 steps per bookkeeping round; refills of 16-128 (at
+# up to 1-16 lanes, and up to 8-64 for the learned order's hot tier (0: as the
+# others); 2-16 steps per bookkeeping round; refills of 16-128 (at
 # least) and 128-1024 (at most) lanes' worth of tasks; 2-9 filter passes per
 # convergence check (a device-driven call runs that many passes); and the
 # level driver: device-driven as a replayed hipGraph (0, the default),
@@ -42,6 +43,7 @@ SPACE = {
     "grid_waves_per_cu": [4, 8],
     "split_steps": [128, 64, 256, 1024, NEVER],
     "group_lanes": [8, 1, 2, 4, 16],
+    "hot_group_lanes": [0, 8, 16, 32, 64],
     "steps_per_round": [8, 2, 4, 16],
     "refill_min": [128, 16, 32, 64],
     "refill_max": [256, 128, 512, 1024],

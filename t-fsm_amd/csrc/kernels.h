@@ -264,6 +264,7 @@ struct VerifyArgs {
     uint32_t long_donate_every;            // its rounds between offers of work (0: never)
     uint32_t long_fanout;                  // records per offer (>= 1)
     uint32_t group_max;                    // most lanes per root task (power of two; <= 1: one)
+    uint32_t hot_group_max;                // the same for the hot tier's tasks
     uint32_t group_base;                   // piece counter of this launch's entry 0 in a lane group
     uint32_t group_cap;                    // entries a launch may run in lane groups
     const uint32_t *hot;                   // with order: entries before *hot (of the first part)

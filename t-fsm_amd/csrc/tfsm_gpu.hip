@@ -985,6 +985,7 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
         va.piece_cnt = c->d_piece_cnt;
         va.piece_incomplete = c->d_piece_inc;
         va.group_max = group_max;
+        va.hot_group_max = opt.hot_group_lanes ? opt.hot_group_lanes : group_max;
         va.group_base = kSplitCap + part * kGroupCap;
         va.group_cap = kGroupCap;
         va.long_steps = 0xffffffffu;
@@ -1098,6 +1099,8 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     if (opt.reserved != 0) throw ArgError{TFSM_ERR_ARGUMENT, "reserved option field must be 0"};
     if (opt.group_lanes > 64 || (opt.group_lanes & (opt.group_lanes - 1)))
         throw ArgError{TFSM_ERR_ARGUMENT, "group_lanes must be 0 or a power of two up to 64"};
+    if (opt.hot_group_lanes > 64 || (opt.hot_group_lanes & (opt.hot_group_lanes - 1)))
+        throw ArgError{TFSM_ERR_ARGUMENT, "hot_group_lanes must be 0 or a power of two up to 64"};
     if (opt.match_order != TFSM_MATCH_ADAPTIVE && opt.match_order != TFSM_MATCH_REFERENCE)
         throw ArgError{TFSM_ERR_ARGUMENT, "unknown matching order"};
     if (opt.steps_per_round > 64) throw ArgError{TFSM_ERR_ARGUMENT, "steps_per_round must be at most 64"};
@@ -1477,6 +1480,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         va.part = 0;
         va.abort_stopped = 0;
         va.group_max = group_max;
+        va.hot_group_max = opt.hot_group_lanes ? opt.hot_group_lanes : group_max;
         va.group_base = kSplitCap;
         va.group_cap = kGroupCap;
         if (total_tasks >= 0xffffffffull) throw ArgError{TFSM_ERR_LIMIT, "more than 2^32 tasks in one call"};

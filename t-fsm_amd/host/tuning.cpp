@@ -26,6 +26,7 @@ uint32_t *field(tfsm_eval_options &o, const std::string &name) {
     if (name == "refill_max") return &o.refill_max;
     if (name == "prune_batch") return &o.prune_batch;
     if (name == "level_mode") return &o.level_mode;
+    if (name == "hot_group_lanes") return &o.hot_group_lanes;
     return nullptr;
 }
 

@@ -144,3 +144,18 @@ def test_mining_level_modes(name, level_mode, workdir):
             assert st["device_calls"] > 0
         assert (st["graph_calls"] > 0) == (level_mode == tfsm.LEVEL_GRAPH)
     ctx.close()
+
+
+@pytest.mark.parametrize("hot", [8, 64])
+def test_hot_tier_groups(hot, workdir):
+    """The learned order's hot tier with up to `hot` lanes per task (other
+    tasks up to 4): exact supports and verdict counts equal the host-driven
+    sequence's with the default groups."""
+    env = env_for("powerlaw100k", 2, 9000, workdir)
+    env.ctx.ranker_load(os.path.join(os.path.dirname(tfsm.__file__), "rankers", "powerlaw.f32"))
+    _, pats = level_patterns(env, "powerlaw100k_f9000_m5", 40)
+    want = env.ctx.eval_level(pats, env.freq, mode=tfsm.MODE_EXACT, learned=True, level_mode=tfsm.LEVEL_HOST)
+    for lm in (tfsm.LEVEL_HOST, tfsm.LEVEL_GRAPH):
+        got = env.ctx.eval_level(pats, env.freq, mode=tfsm.MODE_EXACT, learned=True, level_mode=lm, group_lanes=4,
+                                 hot_group_lanes=hot)
+        same_results(got, want, True, f"hot tier groups of {hot}, level mode {lm}")
