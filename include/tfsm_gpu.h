@@ -151,6 +151,11 @@ typedef struct {
     uint32_t hot_group_lanes;   /* most lanes per task of the learned order's hot tier (the deciding
                                    vertex's tasks of the patterns expected infrequent; 0: group_lanes,
                                    else a power of two <= 64); verdicts do not depend on it */
+    uint32_t ldf_push;          /* a pattern vertex with no parent domain takes its LDF candidates
+                                   from the adjacency of a neighbouring vertex's parent domain,
+                                   instead of its whole label bucket, when a bound on the edges
+                                   walked is at most ldf_push percent of the bucket (0: 100;
+                                   0xffffffff: never); the filtered domains do not depend on it */
 } tfsm_eval_options;
 
 /* How one call's launch sequence is driven.  None changes a result.

@@ -35,9 +35,11 @@ NEVER = 0xffffffff
 # up to 1-16 lanes, and up to 8-64 for the learned order's hot tier (0: as the
 # others); 2-16 steps per bookkeeping round; refills of 16-128 (at
 # least) and 128-1024 (at most) lanes' worth of tasks; 2-9 filter passes per
-# convergence check (a device-driven call runs that many passes); and the
+# convergence check (a device-driven call runs that many passes); the
 # level driver: device-driven as a replayed hipGraph (0, the default),
-# host-driven (1) or device-driven kernel by kernel (2).
+# host-driven (1) or device-driven kernel by kernel (2); and the bound, in
+# percent of a label bucket, under which a new pattern vertex's LDF
+# candidates come from a neighbouring parent domain's adjacency (0: 100).
 SPACE = {
     "level_mode": [0, 1, 2],
     "grid_waves_per_cu": [4, 8],
@@ -48,6 +50,7 @@ SPACE = {
     "refill_min": [128, 16, 32, 64],
     "refill_max": [256, 128, 512, 1024],
     "prune_batch": [6, 2, 4, 9],
+    "ldf_push": [0, NEVER, 25, 400],
 }
 
 

@@ -31,6 +31,8 @@ constexpr int kMaxE = TFSM_MAX_PATTERN_EDGES;
 constexpr int kMaxSlots = 2 * kMaxE;
 constexpr int kBlock = 256;            // threads per block of every kernel
 constexpr int kCutEdges = 12;          // pattern edges of the adaptive matching order
+constexpr uint8_t kNoPush = 0xff;
+constexpr uint32_t kPushItem = 0x80000000u;   // LDF work entry x: a pushed vertex (its items index src[push_x])
 
 struct DevGraph {
     uint32_t n;        // vertices
@@ -79,6 +81,12 @@ struct PatStatic {
     // order (a coalesced read beside src); null for parent domains
     const unsigned long long *src_mask[kMaxV];
     uint32_t src_n[kMaxV];
+    // a label-bucket vertex u whose pattern neighbour push_x[u] has a parent
+    // domain takes its LDF candidates from that domain's neighbours over the
+    // key push_key[u] (label(u) x edge label of x - u) instead of the whole
+    // bucket (kNoPush: from the bucket)
+    uint8_t push_x[kMaxV];
+    uint32_t push_key[kMaxV];
     // domains
     // per 32 data vertices, a pair of words: the membership bits of C(u),
     // then the rank (index in C(u)) of the group's first member, so a table
@@ -290,7 +298,8 @@ struct LevelArgs {
     PatStatic *ps;
     PatState *st;                          // P + 1: the last entry's first word is the filter's change flag
     // filter
-    WorkTable wsrc;                        // (pattern, u, source candidate)
+    WorkTable wldf;                        // LDF: (pattern, u, source item), pushed vertices flagged
+    WorkTable wsrc;                        // DP-iso: (pattern, u, source candidate)
     const uint32_t *ex_off;                // push-down sources (null: none)
     const PushSrc *ex;
     uint32_t *pflags;                      // per DP-iso pass: 1 when it removed a candidate

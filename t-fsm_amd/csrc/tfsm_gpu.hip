@@ -226,9 +226,12 @@ constexpr uint32_t kGraphPrioBlocks = 512;
 constexpr uint32_t kDeviceMaxPatterns = 4096;
 constexpr size_t kDeviceRegionBudget = size_t(4) << 30;
 constexpr uint32_t kNoSlot = 0xffffffffu;
+constexpr uint32_t kFanBins = 32;
+constexpr size_t kFanMaxCells = 1u << 16;     // label x key cells with fan-out bins
+constexpr uint32_t kLdfPushPct = 100;
 // launch grids of a device-driven call, which with the options make a graph's key
 struct LevelShape {
-    uint32_t learned, passes, gfill, gsrc, gpair, gaux, gsample, grank, vgrid, me8;
+    uint32_t learned, passes, gfill, gldf, gsrc, gpair, gaux, gsample, grank, vgrid, me8;
     bool operator==(const LevelShape &o) const { return std::memcmp(this, &o, sizeof(o)) == 0; }
 };
 
@@ -247,6 +250,12 @@ struct tfsm_ctx {
     unsigned long long *d_keymask_bucket = nullptr;  // the same in label-bucket order
     double *d_score = nullptr;               // Fraction-Score table, n x L (computed on first use)
     std::vector<uint32_t> bucket_off;        // host copy, L + 1
+    // fan-out bins for LDF push decisions: for label l and segment key k,
+    // bin b of (l * K + k) * kFanBins counts the vertices of label l with
+    // 2^b .. 2^(b+1) - 1 neighbours over k (fan_n) and sums those counts
+    // (fan_sum); empty when L * K is too large to keep
+    std::vector<uint32_t> fan_n;
+    std::vector<unsigned long long> fan_sum;
     DevGraph g{};
     // scratch
     Arena work;
@@ -333,6 +342,26 @@ int guarded(tfsm_ctx *c, F &&f) {
     } catch (const std::bad_alloc &) {
         return fail(c, TFSM_ERR_LIMIT, "host allocation failed");
     }
+}
+
+// Upper bound on the neighbours over segment key `key` of any `s` vertices of
+// label `lab`: whole fan-out bins from the largest down, then the rest of `s`
+// at the next bin's largest fan-out (at most twice the true sum of the s
+// largest fan-outs).
+unsigned long long fan_bound(const tfsm_ctx *c, uint32_t lab, uint32_t key, uint32_t s) {
+    const size_t cell = ((size_t)lab * c->K + key) * kFanBins;
+    unsigned long long est = 0;
+    for (int b = kFanBins - 1; b >= 0 && s; --b) {
+        const uint32_t n = c->fan_n[cell + b];
+        if (n <= s) {
+            est += c->fan_sum[cell + b];
+            s -= n;
+        } else {
+            est += std::min<unsigned long long>(c->fan_sum[cell + b], (unsigned long long)s * ((2ull << b) - 1));
+            s = 0;
+        }
+    }
+    return est;
 }
 
 uint32_t grid_for(unsigned long long items, int num_cus) {
@@ -749,7 +778,7 @@ void launch_level(tfsm_ctx *c, const LevelShape &sh, bool events) {
     const uint32_t slot = c->level_slot;
     if (events) HIPX(hipEventRecord(c->ev[0], s));
     hipLaunchKernelGGL(k_fills_g, dim3(sh.gfill), dim3(kBlock), 0, s, slot);
-    hipLaunchKernelGGL(k_ldf_g, dim3(sh.gsrc), dim3(kBlock), 0, s, slot);
+    hipLaunchKernelGGL(k_ldf_g, dim3(sh.gldf), dim3(kBlock), 0, s, slot);
     for (uint32_t i = 0; i < sh.passes; ++i) hipLaunchKernelGGL(k_prune_g, dim3(sh.gsrc), dim3(kBlock), 0, s, slot, i);
     hipLaunchKernelGGL(k_compact_g, dim3(sh.gpair), dim3(kBlock), 0, s, slot);    // and the plan
     if (events) HIPX(hipEventRecord(c->ev[1], s));
@@ -867,11 +896,14 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     unsigned long long *aux_base = c->work.alloc<unsigned long long>(M + 1);
     const uint32_t nchg = n * kMaxV;
     uint32_t *d_chg = c->work.alloc<uint32_t>(3 * (size_t)nchg);
-    HostWork src_work;
+    HostWork src_work, ldf_work;
     std::vector<uint32_t> pair_p, pair_u, aux_pat, aux_x;
     for (uint32_t p = 0; p < n; ++p) {
         for (uint32_t u = 0; u < hs[p].nv; ++u) {
             src_work.add(p, u, hs[p].src_n[u]);
+            const uint32_t px = hs[p].push_x[u];
+            if (px == kNoPush) ldf_work.add(p, u, hs[p].src_n[u]);
+            else ldf_work.add(p, u | kPushItem, hs[p].src_n[px]);
             pair_p.push_back(p);
             pair_u.push_back(u);
         }
@@ -884,13 +916,15 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     std::memset(L, 0, sizeof(*L));
     const uint32_t group_max = opt.group_lanes == 0 ? kDefaultGroupLanes : opt.group_lanes;
     const size_t gz = std::min<unsigned long long>(task_ub, kGroupCap);
-    PhaseBatch b(c, n * sizeof(PatStatic) + src_work.bytes() + 4 * ex_off.size() + sizeof(PushSrc) * ex.size() +
+    PhaseBatch b(c, n * sizeof(PatStatic) + src_work.bytes() + ldf_work.bytes() + 4 * ex_off.size() +
+                        sizeof(PushSrc) * ex.size() +
                         8 * pair_p.size() + 8 * aux_pat.size() + 6 * 256 +
                         sizeof(CopyDesc) * ((bits_words + ver_ub) / kFillPiece + (n + 1) * sizeof(PatState) / 4 / kFillPiece +
                                             (size_t)n * kMaxV * (2 + kPrioQ) / kFillPiece + 2 * gz / kFillPiece + 24) +
                         256);
     L->ps = b.add(hs.data(), n);
     L->wsrc = src_work.to_device(b);
+    L->wldf = ldf_work.to_device(b);
     L->ex_off = ex.empty() ? nullptr : b.add(ex_off);
     L->ex = ex.empty() ? nullptr : b.add(ex);
     L->pairs_p = b.add(pair_p);
@@ -1007,6 +1041,7 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     sh.passes = passes;
     sh.gfill = grid_class((unsigned long long)L->nfills * kBlock, 4096);
     sh.gsrc = grid_class(src_work.base.back(), (uint32_t)c->num_cus * 16);
+    sh.gldf = grid_class(ldf_work.base.back(), (uint32_t)c->num_cus * 16);
     sh.gpair = grid_class((unsigned long long)pair_p.size() * kBlock, (uint32_t)c->num_cus * 16);
     // the table build's waves loop over the work (the sizes are known only on
     // the device): two blocks per CU; more resident waves than that made the
@@ -1136,6 +1171,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     // ---------------- host description of every pattern ----------------
     std::vector<PatStatic> hs(n);
     std::vector<uint32_t> slot_of((size_t)n * kMaxV * kMaxV, 0xffffffffu);
+    const uint32_t push_pct = opt.ldf_push == 0 ? kLdfPushPct : opt.ldf_push == 0xffffffffu ? 0 : opt.ldf_push;
     std::vector<char> distinct(n), acyclic(n);
     size_t bits_words = 0, dom_words = 0;
     for (uint32_t p = 0; p < n; ++p) {
@@ -1192,6 +1228,25 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
             }
             bits_words += 2 * (size_t)c->W;
             dom_words += std::max<uint32_t>(P.src_n[u], 1);
+        }
+        // a label-bucket vertex next to a vertex with a parent domain takes
+        // its LDF candidates from that domain's neighbours when a bound on
+        // the neighbours walked is at most push_pct percent of its bucket
+        // (the neighbour with the smallest bound)
+        for (uint32_t u = 0; u < P.nv; ++u) {
+            P.push_x[u] = kNoPush;
+            if (P.src_mask[u] == nullptr || push_pct == 0 || c->fan_n.empty()) continue;
+            unsigned long long best = (unsigned long long)P.src_n[u] * push_pct / 100 + 1;
+            for (uint32_t j = 0; j < P.nbr_n[u]; ++j) {
+                const uint32_t x = P.nbr_v[u][j];
+                if (P.src_mask[x] != nullptr) continue;
+                const uint32_t key = P.nbr_key[u][j] % c->EL + P.vlab[u] * c->EL;
+                const unsigned long long est = fan_bound(c, P.vlab[x], key, P.src_n[x]);
+                if (est >= best) continue;
+                best = est;
+                P.push_x[u] = (uint8_t)x;
+                P.push_key[u] = key;
+            }
         }
         bool dl = true;
         for (uint32_t a = 0; a < P.nv; ++a)
@@ -1271,21 +1326,26 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     }
 
     // ---------------- filter: LDF, DP-iso to the fixpoint, compaction ----------------
-    HostWork src_work;
+    HostWork src_work, ldf_work;
     std::vector<uint32_t> pair_p, pair_u;
     for (uint32_t p = 0; p < n; ++p)
         for (uint32_t u = 0; u < hs[p].nv; ++u) {
             src_work.add(p, u, hs[p].src_n[u]);
+            const uint32_t px = hs[p].push_x[u];
+            if (px == kNoPush) ldf_work.add(p, u, hs[p].src_n[u]);
+            else ldf_work.add(p, u | kPushItem, hs[p].src_n[px]);
             pair_p.push_back(p);
             pair_u.push_back(u);
         }
     // this phase's uploads in one copy, its fills in one launch
-    PhaseBatch b1(c, n * sizeof(PatStatic) + src_work.bytes() + 2 * 256 + 8 * pair_p.size() + 4 * ex_off.size() +
+    PhaseBatch b1(c, n * sizeof(PatStatic) + src_work.bytes() + ldf_work.bytes() + 2 * 256 + 8 * pair_p.size() +
+                         4 * ex_off.size() +
                          sizeof(PushSrc) * ex.size() + 512 +
                          sizeof(CopyDesc) * (bits_words / kFillPiece + (n + 1) * sizeof(PatState) / 4 / kFillPiece + 8) +
                          512);
     PatStatic *d_ps = b1.add(hs.data(), n);
     WorkTable wsrc = src_work.to_device(b1);
+    const WorkTable wldf = ldf_work.to_device(b1);
     const uint32_t *d_ex_off = ex.empty() ? nullptr : b1.add(ex_off);
     const PushSrc *d_ex = ex.empty() ? nullptr : b1.add(ex);
     const uint32_t *d_pair_p = b1.add(pair_p), *d_pair_u = b1.add(pair_u);
@@ -1299,8 +1359,9 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     b1.zero(d_chg, 3 * (size_t)nchg * sizeof(uint32_t));
     b1.flush();
     const uint32_t gsrc = grid_for(wsrc.total, c->num_cus);
-    if (wsrc.total)
-        hipLaunchKernelGGL(k_ldf, dim3(gsrc), dim3(kBlock), 0, s, c->g, d_ps, wsrc, d_ex_off, (const PushSrc *)d_ex);
+    if (wldf.total)
+        hipLaunchKernelGGL(k_ldf, dim3(grid_for(wldf.total, c->num_cus)), dim3(kBlock), 0, s, c->g, d_ps, wldf,
+                           d_ex_off, (const PushSrc *)d_ex);
     HIPX(hipGetLastError());
     // DP-iso passes in batches without host round trips: a pass returns at
     // once when the one before it removed nothing (the fixpoint is reached);
@@ -1731,6 +1792,10 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
         std::vector<uint32_t> nbr(g->num_adj);
         std::vector<uint32_t> cnt(c->K + 1);
         std::vector<unsigned long long> keymask(c->n, 0);
+        if ((size_t)c->L * c->K <= kFanMaxCells) {
+            c->fan_n.assign((size_t)c->L * c->K * kFanBins, 0);
+            c->fan_sum.assign((size_t)c->L * c->K * kFanBins, 0);
+        }
         for (uint32_t v = 0; v < c->n; ++v) {
             const uint32_t b = g->offsets[v], e = g->offsets[v + 1];
             std::fill(cnt.begin(), cnt.end(), 0);
@@ -1745,7 +1810,13 @@ int tfsm_ctx_create(const tfsm_csr *g, tfsm_ctx **out) {
             for (uint32_t k = 0; k < c->K; ++k) {
                 sv[k] = acc;
                 acc += cnt[k + 1];
-                if (cnt[k + 1]) keymask[v] |= 1ull << (k & 63);
+                if (!cnt[k + 1]) continue;
+                keymask[v] |= 1ull << (k & 63);
+                if (!c->fan_n.empty()) {
+                    const size_t cell = ((size_t)g->vlabels[v] * c->K + k) * kFanBins + (31 - __builtin_clz(cnt[k + 1]));
+                    c->fan_n[cell]++;
+                    c->fan_sum[cell] += cnt[k + 1];
+                }
             }
             sv[c->K] = acc;
             std::vector<uint32_t> fill(sv, sv + c->K);

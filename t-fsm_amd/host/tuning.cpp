@@ -27,6 +27,7 @@ uint32_t *field(tfsm_eval_options &o, const std::string &name) {
     if (name == "prune_batch") return &o.prune_batch;
     if (name == "level_mode") return &o.level_mode;
     if (name == "hot_group_lanes") return &o.hot_group_lanes;
+    if (name == "ldf_push") return &o.ldf_push;
     return nullptr;
 }
 

@@ -59,7 +59,7 @@ class EvalOptions(ctypes.Structure):
                 ("num_pushdown", ctypes.c_uint32), ("pushdown", ctypes.c_void_p),
                 ("steps_per_round", ctypes.c_uint32), ("refill_min", ctypes.c_uint32),
                 ("refill_max", ctypes.c_uint32), ("prune_batch", ctypes.c_uint32), ("level_mode", ctypes.c_uint32),
-                ("hot_group_lanes", ctypes.c_uint32)]
+                ("hot_group_lanes", ctypes.c_uint32), ("ldf_push", ctypes.c_uint32)]
 
 
 class PatternResult(ctypes.Structure):
@@ -467,7 +467,7 @@ def load_tuning(path):
         raise TfsmError(-1, host_lib().tfsmh_last_error().decode())
     names = ["grid_waves_per_cu", "split_steps", "split_fanout", "long_steps", "long_waves_per_cu",
              "long_donate_every", "long_fanout", "match_order", "group_lanes", "steps_per_round", "refill_min",
-             "refill_max", "prune_batch", "level_mode", "hot_group_lanes"]
+             "refill_max", "prune_batch", "level_mode", "hot_group_lanes", "ldf_push"]
     return {k: getattr(opt, k) for k in names if getattr(opt, k)}
 
 

@@ -159,3 +159,22 @@ def test_hot_tier_groups(hot, workdir):
         got = env.ctx.eval_level(pats, env.freq, mode=tfsm.MODE_EXACT, learned=True, level_mode=lm, group_lanes=4,
                                  hot_group_lanes=hot)
         same_results(got, want, True, f"hot tier groups of {hot}, level mode {lm}")
+
+
+@pytest.mark.parametrize("level_mode", [tfsm.LEVEL_HOST, tfsm.LEVEL_GRAPH], ids=["host", "graph"])
+@pytest.mark.parametrize("name", ["gnm2k_f40_m4", "powerlaw100k_f9000_m5"])
+def test_ldf_push_mining(name, level_mode, workdir):
+    """Every new pattern vertex next to a parent domain takes its LDF
+    candidates from that domain's adjacency (ldf_push 1000000): the mined
+    pattern set equals the reference's, in both level drivers."""
+    m = golden_manifest()[name]
+    path = workload_path(m["workload"], m["seed"], workdir)
+    g = tfsm.Graph(path, m["freq"])
+    ctx = tfsm.Context(g)
+    kp = os.path.join(str(workdir), f"{name}_push{level_mode}.keys")
+    print(f"{name} push always, level mode {level_mode}", flush=True)
+    n, st = ctx.mine(m["freq"], m["max_nodes"], threads=4, batched=True, keys_path=kp, level_mode=level_mode,
+                     ldf_push=1000000)
+    keys = set(l.split("\t")[0] for l in open(kp) if l.strip())
+    assert n == m["reported_total"] and keys == golden_keys(name)
+    ctx.close()

@@ -135,8 +135,14 @@ def test_capped_verdicts(name, seed, freq, golden, workdir):
             assert res.support_upper < freq
 
 
-def test_parent_domains(workdir):
-    """A child's candidates start from the parent's domains minus its proven-invalid ones."""
+# ldf_push: 0 is the library's bound (push when at most 100% of the bucket),
+# 1000000 pushes from every neighbouring parent domain, 0xffffffff never does
+@pytest.mark.parametrize("push", [0, 1000000, 0xffffffff], ids=["default", "always", "never"])
+def test_parent_domains(push, workdir):
+    """A child's candidates start from the parent's domains minus its
+    proven-invalid ones; a new vertex's candidates come from its label bucket
+    or from the adjacency of its neighbour's parent domain, with the same
+    filtered domains either way."""
     env = env_for("gnm2k", 1, 40, workdir)
     keys = sorted(k for k in golden_keys("gnm2k_f40_m4") if k.count(";") == 2)
     checked = 0
@@ -152,7 +158,7 @@ def test_parent_domains(workdir):
         # child: forward edge from the last vertex to a new vertex with label vl[0]
         cvl, ces = vl + [vl[0]], es + [(len(vl) - 1, len(vl), es[0][2])]
         child = env.dense(cvl, ces)
-        cres = env.ctx.eval_pattern(child, env.freq, mode=tfsm.MODE_EXACT, parent=pres.domain_handle)
+        cres = env.ctx.eval_pattern(child, env.freq, mode=tfsm.MODE_EXACT, parent=pres.domain_handle, ldf_push=push)
         cref = env.og.eval(cvl, ces, env.freq, mode=1, parent_domains=pdoms)
         for u in range(len(cvl)):
             assert np.array_equal(env.ctx.last_domain(u), cref.domain(u)), f"{cvl} {ces} domain {u}"
