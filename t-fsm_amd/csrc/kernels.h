@@ -96,6 +96,11 @@ struct PatStatic {
     uint32_t dom_size[kMaxV];           // filled by the host once known
     uint32_t *valid_bits[kMaxV];
     uint32_t *invalid_bits[kMaxV];
+    // a device-driven call's copies kept after it (retained domains): the
+    // domain and its invalid bits, written by the call's last kernel (null:
+    // the host copies what it keeps after the call)
+    uint32_t *keep_dom[kMaxV];
+    uint32_t *keep_inv[kMaxV];
     // directed pattern edges
     uint8_t slot_a[kMaxSlots], slot_b[kMaxSlots];
     uint32_t slot_key[kMaxSlots];

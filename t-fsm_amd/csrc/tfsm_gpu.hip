@@ -226,12 +226,19 @@ constexpr uint32_t kGraphPrioBlocks = 512;
 constexpr uint32_t kDeviceMaxPatterns = 4096;
 constexpr size_t kDeviceRegionBudget = size_t(4) << 30;
 constexpr uint32_t kNoSlot = 0xffffffffu;
+constexpr unsigned long long kDeviceKeepBudget = 256ull << 20;   // bytes of kept-domain buffers per call
+// a device-driven call's read-back block: the phase clock, two search
+// counters, four words (pool used, overflow, filter passes run, compaction
+// blocks done), then the pattern states
+constexpr size_t kOutCtr = 5 * kClockWays * sizeof(unsigned long long);
+constexpr size_t kOutFlags = kOutCtr + 2 * sizeof(SearchCounters);
+constexpr size_t kOutStates = (kOutFlags + 4 * sizeof(uint32_t) + 255) & ~(size_t)255;
 constexpr uint32_t kFanBins = 32;
 constexpr size_t kFanMaxCells = 1u << 16;     // label x key cells with fan-out bins
 constexpr uint32_t kLdfPushPct = 100;
 // launch grids of a device-driven call, which with the options make a graph's key
 struct LevelShape {
-    uint32_t learned, passes, gfill, gldf, gsrc, gpair, gaux, gsample, grank, vgrid, me8;
+    uint32_t learned, passes, gfill, gldf, gsrc, gpair, gaux, gsample, grank, vgrid, me8, keep;
     bool operator==(const LevelShape &o) const { return std::memcmp(this, &o, sizeof(o)) == 0; }
 };
 
@@ -294,9 +301,10 @@ struct tfsm_ctx {
     std::string err;
     // device-driven level calls
     uint32_t level_slot = kNoSlot;          // this context's descriptor in c_level
+    bool stream_idle = false;               // no work queued since the last eval call's final sync
     LevelArgs *d_level = nullptr;           // its device address
     Pinned<LevelArgs> h_level;              // the descriptor's staging
-    Pinned<unsigned long long> h_clk;       // the phase clock read back
+    Pinned<unsigned char> h_out;            // a device-driven call's read-back block
     int wall_khz = 100000;                  // rate of the device's wall clock
     std::vector<std::pair<LevelShape, hipGraphExec_t>> graphs;   // captured launch sequences
 };
@@ -331,8 +339,11 @@ int fail(tfsm_ctx *c, int code, const std::string &msg) {
     return code;
 }
 
+// `eval`: an eval entry point, which reads the idle flag itself; every other
+// entry point may queue work on the stream, so it clears the flag
 template <class F>
-int guarded(tfsm_ctx *c, F &&f) {
+int guarded(tfsm_ctx *c, F &&f, bool eval = false) {
+    if (!eval) c->stream_idle = false;
     try {
         return f();
     } catch (const HipError &e) {
@@ -605,6 +616,20 @@ double frac_value(const FracSum &f) {
 
 // Stats, per-pattern results and retained child domains of a finished call,
 // from the states, counters and flags read back (both drivers).
+// Buffers a device-driven call's last kernel filled with every pattern's
+// domains (one per pattern, null when the host copies what it keeps):
+// finish_call takes those of the patterns it retains, and the rest go back
+// to the cache when the call ends, on every path.
+struct KeptBuffers {
+    tfsm_ctx *c;
+    std::vector<std::pair<void *, size_t>> buf;
+    explicit KeptBuffers(tfsm_ctx *c_) : c(c_) {}
+    ~KeptBuffers() {
+        for (auto &b : buf)
+            if (b.first) c->cache.put(b.first, b.second);
+    }
+};
+
 struct Finished {
     std::vector<PatStatic> &hs;
     const PatState *hst;
@@ -616,6 +641,7 @@ struct Finished {
     uint32_t verify_launches;
     uint32_t level_mode;
     bool timed;                     // the stream events of every phase were recorded
+    KeptBuffers *kept;              // the device's copies of every domain, or null
 };
 
 int finish_call(tfsm_ctx *c, const EvalCall &call, const Finished &f) {
@@ -716,7 +742,22 @@ int finish_call(tfsm_ctx *c, const EvalCall &call, const Finished &f) {
         } else {
             r.frequent = decided[p] != TFSM_DECIDED_FILTER && lo >= call.freq;
         }
-        if (r.frequent && opt.retain_domains) {
+        if (r.frequent && opt.retain_domains && f.kept) {
+            // the call's last kernel copied the domain here
+            Retained R;
+            R.nv = P.nv;
+            R.buf = f.kept->buf[p].first;
+            R.cls = f.kept->buf[p].second;
+            f.kept->buf[p].first = nullptr;
+            for (uint32_t u = 0; u < P.nv; ++u) {
+                R.vlab[u] = P.vlab[u];
+                R.size[u] = P.dom_size[u];
+                R.dom[u] = P.keep_dom[u];
+                R.invalid[u] = P.keep_inv[u];
+            }
+            r.domain_handle = c->next_handle++;
+            c->retained.emplace(r.domain_handle, R);
+        } else if (r.frequent && opt.retain_domains) {
             size_t words = 0;
             for (uint32_t u = 0; u < P.nv; ++u) words += P.dom_size[u] + (P.dom_size[u] + 31) / 32;
             Retained R;
@@ -806,6 +847,7 @@ void launch_level(tfsm_ctx *c, const LevelShape &sh, bool events) {
         HIPX(hipEventRecord(c->ev[5], s));
         HIPX(hipEventRecord(c->ev[3], s));
     }
+    if (sh.keep) hipLaunchKernelGGL(k_keep_g, dim3(sh.gpair), dim3(kBlock), 0, s, slot);
     HIPX(hipGetLastError());
 }
 
@@ -912,8 +954,32 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
             aux_x.push_back(sl);
         }
     }
+    // retained domains: when their bounds fit, every pattern gets a buffer
+    // for its domains at their source sizes, filled by the call's last
+    // kernel, so the call ends with no copies left to launch
+    KeptBuffers kept(c);
+    if (opt.retain_domains) {
+        unsigned long long keep_words = 0;
+        for (uint32_t p = 0; p < n; ++p)
+            for (uint32_t u = 0; u < hs[p].nv; ++u) keep_words += hs[p].src_n[u] + (hs[p].src_n[u] + 31ull) / 32;
+        if (keep_words * 4 <= kDeviceKeepBudget) {
+            kept.buf.resize(n);
+            for (uint32_t p = 0; p < n; ++p) {
+                PatStatic &P = hs[p];
+                size_t words = 0;
+                for (uint32_t u = 0; u < P.nv; ++u) words += P.src_n[u] + (P.src_n[u] + 31) / 32;
+                kept.buf[p].first = c->cache.get(std::max<size_t>(words, 1) * sizeof(uint32_t), &kept.buf[p].second);
+                uint32_t *w = static_cast<uint32_t *>(kept.buf[p].first);
+                for (uint32_t u = 0; u < P.nv; ++u) {
+                    P.keep_dom[u] = w; w += P.src_n[u];
+                    P.keep_inv[u] = w; w += (P.src_n[u] + 31) / 32;
+                }
+            }
+        }
+    }
     LevelArgs *L = c->h_level.ensure(1);
     std::memset(L, 0, sizeof(*L));
+    const size_t out_bytes = kOutStates + (n + 1) * sizeof(PatState);
     const uint32_t group_max = opt.group_lanes == 0 ? kDefaultGroupLanes : opt.group_lanes;
     const size_t gz = std::min<unsigned long long>(task_ub, kGroupCap);
     PhaseBatch b(c, n * sizeof(PatStatic) + src_work.bytes() + ldf_work.bytes() + 4 * ex_off.size() +
@@ -921,7 +987,21 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
                         8 * pair_p.size() + 8 * aux_pat.size() + 6 * 256 +
                         sizeof(CopyDesc) * ((bits_words + ver_ub) / kFillPiece + (n + 1) * sizeof(PatState) / 4 / kFillPiece +
                                             (size_t)n * kMaxV * (2 + kPrioQ) / kFillPiece + 2 * gz / kFillPiece + 24) +
-                        256);
+                        out_bytes + 256);
+    // the block read back after the call, uploaded with the batch: the phase
+    // clock (its minimum slots at the largest value), both search launches'
+    // counters, the pool, overflow and pass words, and the pattern states
+    // (with the filter's last change flag after them), otherwise zeros
+    unsigned char *d_out;
+    {
+        std::vector<unsigned long long> out0((out_bytes + 7) / 8, 0ull);
+        for (uint32_t w = 0; w < kClockWays; ++w) out0[w] = out0[3 * kClockWays + w] = ~0ull;
+        d_out = reinterpret_cast<unsigned char *>(b.add(out0));
+    }
+    unsigned long long *d_clk = reinterpret_cast<unsigned long long *>(d_out);
+    SearchCounters *d_ctr = reinterpret_cast<SearchCounters *>(d_out + kOutCtr);
+    uint32_t *d_flags = reinterpret_cast<uint32_t *>(d_out + kOutFlags);
+    d_st = reinterpret_cast<PatState *>(d_out + kOutStates);
     L->ps = b.add(hs.data(), n);
     L->wsrc = src_work.to_device(b);
     L->wldf = ldf_work.to_device(b);
@@ -932,13 +1012,9 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     L->aux_pat = b.add(aux_pat);
     L->aux_x = b.add(aux_x);
     b.zero(bits_region, bits_words * sizeof(uint32_t));
-    b.zero(d_st, (n + 1) * sizeof(PatState));
-    // pool used, pool overflow, filter passes run, compaction blocks done
-    b.zero(c->d_flags, 4 * sizeof(uint32_t));
     b.zero(c->d_pflags, passes * sizeof(uint32_t));
     b.zero(d_chg, 3 * (size_t)nchg * sizeof(uint32_t));
     if (ver_ub) b.zero(verdict_region, ver_ub * sizeof(uint32_t));
-    b.zero(c->d_ctr, 2 * sizeof(SearchCounters));
     if (group_max > 1 && gz) {
         b.zero(c->d_piece_cnt + kSplitCap, gz * sizeof(uint32_t));
         b.zero(c->d_piece_cnt + kSplitCap + kGroupCap, gz * sizeof(uint32_t));
@@ -951,10 +1027,7 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     L->nfills = (uint32_t)b.zeros.size();
     L->fills = b.add(b.zeros);
     b.zeros.clear();
-    // the phase clock's starting values (min slots at the largest value)
-    std::vector<unsigned long long> clk0(5 * kClockWays, 0ull);
-    for (uint32_t w = 0; w < kClockWays; ++w) clk0[w] = clk0[3 * kClockWays + w] = ~0ull;
-    L->clk = b.add(clk0);
+    L->clk = d_clk;
     L->P = n;
     L->g = c->g;
     L->st = d_st;
@@ -963,16 +1036,16 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     L->nchg = nchg;
     L->passes = passes;
     L->npairs = (uint32_t)pair_p.size();
-    L->passes_run = c->d_flags + 2;
-    L->blocks_done = c->d_flags + 3;
+    L->passes_run = d_flags + 2;
+    L->blocks_done = d_flags + 3;
     L->aux_region = aux_region;
     L->verdict_region = verdict_region;
     L->task_base = task_base;
     L->aux_base = aux_base;
     L->aux_M = (uint32_t)M;
     L->pool = c->d_pool;
-    L->pool_used = c->d_flags;
-    L->overflow = c->d_flags + 1;
+    L->pool_used = d_flags;
+    L->overflow = d_flags + 1;
     L->pool_cap = c->pool_cap;
     uint32_t *hist = nullptr;
     if (learned) {
@@ -1003,11 +1076,11 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
         va.task_base = task_base;
         va.P = n;
         va.epoch = ++c->epoch;
-        va.ctr = c->d_ctr + part;
+        va.ctr = d_ctr + part;
         va.pool = c->d_pool;
         va.aux_region = aux_region;
         va.max_edges = max_edges;
-        va.overflow = c->d_flags + 1;
+        va.overflow = d_flags + 1;
         va.steps_per_round = opt.steps_per_round ? opt.steps_per_round : kDefaultStepsPerRound;
         va.refill_min = opt.refill_min ? opt.refill_min : kDefaultRefillMin;
         va.refill_max = opt.refill_max ? opt.refill_max : kDefaultRefillMax;
@@ -1053,6 +1126,7 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     sh.grank = grid_class(task_ub, (uint32_t)c->num_cus * 8);
     sh.vgrid = (uint32_t)c->num_cus * per_cu;
     sh.me8 = max_edges <= 8;
+    sh.keep = !kept.buf.empty();
     // the batch's uploads, then the descriptor, then the sequence
     b.flush();
     HIPX(hipMemcpyAsync(c->d_level, L, sizeof(LevelArgs), hipMemcpyHostToDevice, s));
@@ -1062,16 +1136,14 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     const bool graph = lm == TFSM_LEVEL_GRAPH;
     if (graph) HIPX(hipGraphLaunch(level_graph(c, sh), s));
     else launch_level(c, sh, true);
-    // one readback: the states (with the filter's change flag), then the
-    // counters of both search launches and the pool and pass words
-    PatState *hst = c->h_state.ensure(n + 1);
-    HIPX(hipMemcpyAsync(hst, d_st, (n + 1) * sizeof(PatState), hipMemcpyDeviceToHost, s));
-    SearchCounters *hctr = c->h_ctr.ensure(3);
-    HIPX(hipMemcpyAsync(hctr, c->d_ctr, 2 * sizeof(SearchCounters) + 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    const uint32_t *hpool = reinterpret_cast<const uint32_t *>(hctr + 2);
-    unsigned long long *hclk = c->h_clk.ensure(5 * kClockWays);
-    if (graph)
-        HIPX(hipMemcpyAsync(hclk, L->clk, 5 * kClockWays * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    // one readback: the clock, the counters, the pool and pass words and the
+    // states (with the filter's change flag)
+    unsigned char *hout = c->h_out.ensure(out_bytes);
+    HIPX(hipMemcpyAsync(hout, d_out, out_bytes, hipMemcpyDeviceToHost, s));
+    const unsigned long long *hclk = reinterpret_cast<const unsigned long long *>(hout);
+    const SearchCounters *hctr = reinterpret_cast<const SearchCounters *>(hout + kOutCtr);
+    const uint32_t *hpool = reinterpret_cast<const uint32_t *>(hout + kOutFlags);
+    const PatState *hst = reinterpret_cast<const PatState *>(hout + kOutStates);
     sync_spin(c);
     if (*reinterpret_cast<const uint32_t *>(hst + n)) return kRedoHost;
     if (hpool[1]) {
@@ -1098,7 +1170,10 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
         }
     }
     const int rc = finish_call(c, call, Finished{hs, hst, hctr, hpool, decided, nullptr, total_tasks,
-                                                 learned ? 2u : 1u, lm, !graph});
+                                                 learned ? 2u : 1u, lm, !graph, sh.keep ? &kept : nullptr});
+    // the kept domains were written in the sequence, so nothing was queued
+    // after the sync unless the call retained nothing that way
+    c->stream_idle = sh.keep || !opt.retain_domains;
     if (graph) {
         // the phase clock in ms: first block .. compaction .. tables .. search start .. end
         const double per_ms = (double)c->wall_khz;
@@ -1163,7 +1238,10 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
     }
     c->work.reset();
     hipStream_t s = c->stream;
-    sync_spin(c);                       // nothing of an earlier call still reads the staging
+    // nothing of an earlier call still reads the staging (a device-driven
+    // call that queued nothing after its final sync left the stream idle)
+    if (!c->stream_idle) sync_spin(c);
+    c->stream_idle = false;
     c->stage.reset();
     if (fraction) ensure_scores(c);
     HIPX(hipEventRecord(c->ev[0], s));
@@ -1719,7 +1797,7 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         return run_eval(c, call, attempt + 1);
     }
     return finish_call(c, call, Finished{hs, hst, hctr, hpool, decided, hfrac, total_tasks, verify_launches,
-                                         TFSM_LEVEL_HOST, true});
+                                         TFSM_LEVEL_HOST, true, nullptr});
 }
 
 }  // namespace
@@ -1939,7 +2017,7 @@ int tfsm_ctx_destroy(tfsm_ctx *c) {
 int tfsm_eval_pattern(tfsm_ctx *ctx, const tfsm_pattern *p, uint64_t parent_handle, uint32_t freq,
                       const tfsm_eval_options *opt, tfsm_pattern_result *out) {
     if (!ctx || !p || !opt || !out) return TFSM_ERR_ARGUMENT;
-    return guarded(ctx, [&]() { return run_eval(ctx, EvalCall{p, &parent_handle, 1, freq, *opt, out}); });
+    return guarded(ctx, [&]() { return run_eval(ctx, EvalCall{p, &parent_handle, 1, freq, *opt, out}); }, true);
 }
 
 // A level whose candidate bitmaps (two words per 32 data vertices per
@@ -1999,7 +2077,7 @@ int tfsm_eval_level(tfsm_ctx *ctx, const tfsm_pattern *ps, const uint64_t *paren
         }
         if (slices > 1) ctx->stats = sum;
         return TFSM_OK;
-    });
+    }, true);
 }
 
 int64_t tfsm_fraction_scores(tfsm_ctx *ctx, double *out, uint64_t cap) {

@@ -19,7 +19,10 @@ namespace tfsm {
 // several levels, and starting threads for each of them cost more than the
 // growth of a small level.  Each job has its own state; the caller takes
 // items too and waits only for the items, so a worker that wakes late finds
-// its job drained and does nothing.
+// its job drained and does nothing.  A new job wakes one worker, and each
+// worker that takes the job wakes the next before working on it, so the
+// caller pays for one wake-up rather than one per worker (it starts GPU work
+// right after posting a background job).
 class WorkerPool {
 public:
     static WorkerPool &get() {
@@ -46,7 +49,7 @@ public:
             cur_ = job;
             ++gen_;
         }
-        cv_.notify_all();
+        cv_.notify_one();
         drain(*job);
         std::unique_lock<std::mutex> lk(job->m);
         job->cv.wait(lk, [&]() { return job->left.load() == 0; });
@@ -79,7 +82,7 @@ public:
             cur_ = job;
             ++gen_;
         }
-        cv_.notify_all();
+        cv_.notify_one();
         return job;
     }
     void finish(const std::shared_ptr<Job> &job) {
@@ -116,6 +119,7 @@ private:
                 seen = gen_;
                 job = cur_;
             }
+            cv_.notify_one();           // the next waiting worker
             if (job) drain(*job);
         }
     }

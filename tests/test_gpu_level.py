@@ -178,3 +178,42 @@ def test_ldf_push_mining(name, level_mode, workdir):
     keys = set(l.split("\t")[0] for l in open(kp) if l.strip())
     assert n == m["reported_total"] and keys == golden_keys(name)
     ctx.close()
+
+
+@pytest.mark.parametrize("level_mode", DEVICE_MODES, ids=MODE_IDS)
+@pytest.mark.parametrize("mode", [tfsm.MODE_EXACT, tfsm.MODE_CAPPED], ids=["exact", "capped"])
+def test_level_retained_domains(mode, level_mode, workdir):
+    """Retained domains of a device-driven level (copied by the sequence's
+    last kernel into buffers set aside before the call) read back, in exact
+    mode, equal to the host-driven call's and to the oracle's child domains
+    (the domain minus the candidates proven invalid).  A capped search stops
+    once the pattern is decided, so its invalid set depends on the schedule:
+    the read-back lies between the exact child domain and the whole domain.
+    Two calls in a row, so the second takes buffers the first released."""
+    env = env_for("gnm2k", 1, 40, workdir)
+    cases, pats = level_patterns(env, "gnm2k_f40_m4", 120)
+    for rnd in range(2):
+        host = env.ctx.eval_level(pats, env.freq, mode=mode, retain=True, level_mode=tfsm.LEVEL_HOST)
+        # enough DP-iso passes in the sequence that the call is not redone host-driven
+        dev = env.ctx.eval_level(pats, env.freq, mode=mode, retain=True, level_mode=level_mode, prune_batch=32)
+        assert env.ctx.last_stats().level_mode == level_mode
+        kept = 0
+        for (vl, es), h, d in zip(cases, host, dev):
+            assert bool(h.frequent) == bool(d.frequent)
+            if not d.frequent:
+                assert d.domain_handle == 0
+                continue
+            ref = env.og.eval(vl, es, env.freq, mode=1)
+            for u in range(len(vl)):
+                got = list(env.ctx.domain_read(d.domain_handle, u))
+                w = f"{vl} {es} vertex {u}"
+                if mode == tfsm.MODE_EXACT:
+                    assert got == list(env.ctx.domain_read(h.domain_handle, u)), w
+                    assert got == list(ref.child_domain(u)), w
+                else:
+                    assert got == sorted(got) and set(ref.child_domain(u)) <= set(got) <= set(ref.domain(u)), w
+            kept += 1
+        for r in host + dev:
+            if r.domain_handle:
+                env.ctx.domain_release(r.domain_handle)
+        assert kept > 10, rnd
