@@ -131,3 +131,34 @@ def test_powerlaw_f6000_m6_pattern_sets(learned, workdir):
     assert set(k for k in keys if k.count(";") > 1) == golden_keys(name)
     assert len(keys) - len(golden_keys(name)) == 4
     ctx.close()
+
+
+def test_rmat20_f5300_through_csr_cache(workdir):
+    """R-MAT scale 20 seed 4 at freq 5300 (45,089 candidates over two levels,
+    658 frequent) on the HBM-resident graph: the first load parses the file
+    and writes the binary CSR cache, the second reads it; batched, then
+    batched with the R-MAT ranker (trained on seeds 11 and 12), each find
+    the reference's pattern set (parallel build; the golden file holds the
+    patterns of three or more vertices)."""
+    name = "rmat20_f5300_m5_parallel"
+    m = golden_manifest()[name]
+    path = workload_path(m["workload"], m["seed"], workdir)
+    for learned in (False, True):
+        print(f"{name}: loading (cache), learned={learned}", flush=True)
+        g = tfsm.Graph(path, m["freq"], cache=True)
+        if learned:
+            assert g.cache_hit
+        ctx = tfsm.Context(g)
+        if learned:
+            ctx.ranker_load(os.path.join(os.path.dirname(tfsm.__file__), "rankers", "rmat.f32"))
+        kp = os.path.join(str(workdir), f"{name}_{int(learned)}.keys")
+        n, stats = ctx.mine(m["freq"], m["max_nodes"], threads=4, batched=True, learned=learned,
+                            mode=tfsm.MODE_CAPPED, keys_path=kp)
+        print(f"{name}: learned={learned} evaluated={stats['evaluated']} mining_s={stats['mining_s']:.3f}",
+              flush=True)
+        assert n == m["reported_total"]
+        keys = set(read_keys(kp))
+        assert set(k for k in keys if k.count(";") > 1) == golden_keys(name)
+        ctx.close()
+        g.close()
+    assert os.path.exists(path + f".f{m['freq']}.csr")
