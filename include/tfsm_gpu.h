@@ -154,8 +154,10 @@ typedef struct {
     uint32_t ldf_push;          /* a pattern vertex with no parent domain takes its LDF candidates
                                    from the adjacency of a neighbouring vertex's parent domain,
                                    instead of its whole label bucket, when a bound on the edges
-                                   walked is at most ldf_push percent of the bucket (0: 100;
-                                   0xffffffff: never); the filtered domains do not depend on it */
+                                   walked is at most ldf_push percent of the bucket (0: 100,
+                                   in calls whose pushed buckets hold at least 64 items per
+                                   thread of one block per CU; 0xffffffff: never); the
+                                   filtered domains do not depend on it */
 } tfsm_eval_options;
 
 /* How one call's launch sequence is driven.  None changes a result.

@@ -236,6 +236,7 @@ constexpr size_t kOutStates = (kOutFlags + 4 * sizeof(uint32_t) + 255) & ~(size_
 constexpr uint32_t kFanBins = 32;
 constexpr size_t kFanMaxCells = 1u << 16;     // label x key cells with fan-out bins
 constexpr uint32_t kLdfPushPct = 100;
+constexpr uint32_t kPushMinItemsPerThread = 64;   // bucket items per thread of a one-block-per-CU grid
 // launch grids of a device-driven call, which with the options make a graph's key
 struct LevelShape {
     uint32_t learned, passes, gfill, gldf, gsrc, gpair, gaux, gsample, grank, vgrid, me8, keep;
@@ -1347,6 +1348,21 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
         }
         distinct[p] = dl;
         acyclic[p] = tp.num_edges == tp.num_vertices - 1;
+    }
+    // by default, pushing is kept to calls whose pushed buckets hold many
+    // items per GPU thread: a call of a few patterns checks every bucket item
+    // on its own thread in about one round, while a pushed item walks a whole
+    // adjacency segment (thousands of entries at a hub), so there the walks'
+    // latency costs more than the bucket's work (R-MAT scale 20, one pattern
+    // per call: 4x the filter time)
+    if (opt.ldf_push == 0) {
+        unsigned long long pushed_items = 0;
+        for (uint32_t p = 0; p < n; ++p)
+            for (uint32_t u = 0; u < hs[p].nv; ++u)
+                if (hs[p].push_x[u] != kNoPush) pushed_items += hs[p].src_n[u];
+        if (pushed_items < (unsigned long long)c->num_cus * kBlock * kPushMinItemsPerThread)
+            for (uint32_t p = 0; p < n; ++p)
+                for (uint32_t u = 0; u < hs[p].nv; ++u) hs[p].push_x[u] = kNoPush;
     }
     // one region for all bitmaps (zeroed at once) and one for all lists
     uint32_t *bits_region = c->work.alloc<uint32_t>(bits_words);
