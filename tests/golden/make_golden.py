@@ -8,6 +8,15 @@ line) in tests/golden/<name>.keys, with tests/golden/manifest.json recording
 the command, the reference's own reported count and a sha256 of the keys.
 
     python tests/golden/make_golden.py [name ...]
+    python tests/golden/make_golden.py --oracle-check name [threads]
+
+The second form cross-checks a fixture with this build's CPU oracle
+(oracle/oracle.cpp, capped mining with the given threads): it mines the same
+configuration, compares the oracle's pattern set (restricted to what the
+fixture holds) and total with the fixture's, and records the outcome under
+"oracle_check" in the fixture's manifest entry.  Fixtures from the parallel
+build, which can drop patterns on its timeout path (parallel/comper.h:1450),
+are pinned this way where the serial build takes too long.
 
 The serial build writes grami_results.txt (serial/grami.h:622-635) with every
 frequent pattern.  The parallel build prints patterns of three or more
@@ -124,6 +133,39 @@ def graph_file(workload, seed, tmp):
     return path
 
 
+def update_manifest(name, entry):
+    manifest_path = os.path.join(HERE, "manifest.json")
+    # several invocations may run at once: merge into the file under a lock
+    with open(manifest_path + ".lock", "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        disk = json.load(open(manifest_path)) if os.path.exists(manifest_path) else {}
+        disk[name] = entry
+        with open(manifest_path, "w") as f:
+            json.dump(disk, f, indent=1, sort_keys=True)
+
+
+def oracle_check(name, threads):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as orc
+    manifest = json.load(open(os.path.join(HERE, "manifest.json")))
+    entry = manifest[name]
+    want = set(l.strip() for l in open(os.path.join(HERE, name + ".keys")) if l.strip())
+    with tempfile.TemporaryDirectory() as tmp:
+        lg = graph_file(entry["workload"], entry["seed"], tmp)
+        t0 = time.time()
+        og = orc.OracleGraph(lg, entry["freq"])
+        out = os.path.join(tmp, "oracle.keys")
+        og.mine(entry["freq"], entry["max_nodes"], threads, 0, out)
+        got = set(l.rstrip("\n").split("\t")[0] for l in open(out) if l.strip())
+        seconds = round(time.time() - t0, 2)
+    held = got if entry["keys_hold"] == "all frequent patterns" else set(k for k in got if k.count(";") > 1)
+    entry["oracle_check"] = dict(threads=threads, seconds=seconds, total=len(got), keys=len(held),
+                                 equal=held == want and len(got) == entry["reported_total"])
+    print(name, entry["oracle_check"])
+    update_manifest(name, entry)
+    return 0 if entry["oracle_check"]["equal"] else 1
+
+
 def main(names):
     manifest_path = os.path.join(HERE, "manifest.json")
     manifest = json.load(open(manifest_path)) if os.path.exists(manifest_path) else {}
@@ -159,14 +201,10 @@ def main(names):
                                   sha256=hashlib.sha256(body.encode()).hexdigest(),
                                   seconds=round(time.time() - t0, 2))
             print(name, manifest[name])
-            # several invocations may run at once: merge into the file under a lock
-            with open(manifest_path + ".lock", "w") as lk:
-                fcntl.flock(lk, fcntl.LOCK_EX)
-                disk = json.load(open(manifest_path)) if os.path.exists(manifest_path) else {}
-                disk[name] = manifest[name]
-                with open(manifest_path, "w") as f:
-                    json.dump(disk, f, indent=1, sort_keys=True)
+            update_manifest(name, manifest[name])
 
 
 if __name__ == "__main__":
+    if sys.argv[1:2] == ["--oracle-check"]:
+        sys.exit(oracle_check(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 8))
     main(sys.argv[1:])
