@@ -1474,9 +1474,12 @@ int run_eval(tfsm_ctx *c, const EvalCall &call, int attempt = 0) {
                                d_chg + (size_t)(passes % 3) * nchg, d_chg + (size_t)((passes + 1) % 3) * nchg, nchg,
                                c->d_flags + 2);
         }
-        hipLaunchKernelGGL(k_compact, dim3((uint32_t)pair_p.size()), dim3(kBlock), 0, s, c->g, d_ps, d_st, d_pair_p,
-                           d_pair_u, passes ? c->d_pflags + passes - 1 : (const uint32_t *)nullptr, d_more,
-                           (uint32_t)(pair_p.size() >= kCompactStageMinPairs));
+        if (pair_p.size() >= kCompactStageMinPairs)
+            hipLaunchKernelGGL(k_compact<true>, dim3((uint32_t)pair_p.size()), dim3(kBlock), 0, s, c->g, d_ps, d_st,
+                               d_pair_p, d_pair_u, passes ? c->d_pflags + passes - 1 : (const uint32_t *)nullptr, d_more);
+        else
+            hipLaunchKernelGGL(k_compact<false>, dim3((uint32_t)pair_p.size()), dim3(kBlock), 0, s, c->g, d_ps, d_st,
+                               d_pair_p, d_pair_u, passes ? c->d_pflags + passes - 1 : (const uint32_t *)nullptr, d_more);
         HIPX(hipGetLastError());
         HIPX(hipMemcpyAsync(hst, d_st, (n + 1) * sizeof(PatState), hipMemcpyDeviceToHost, s));
         sync_spin(c);
