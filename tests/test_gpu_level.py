@@ -122,6 +122,32 @@ def test_unconverged_filter_is_redone(workdir):
     assert redone >= 1
 
 
+def test_redone_level_retains_domains(workdir):
+    """A device-driven call that keeps domains sets buffers aside before it
+    runs; when its filter does not reach the fixpoint (prune_batch 1) the
+    call is redone host-driven, the buffers go back, and the retained
+    domains are the host-driven call's, read back equal."""
+    env = env_for("powerlaw100k", 2, 9000, workdir)
+    _, pats = level_patterns(env, "powerlaw100k_f9000_m5", 40)
+    for rnd in range(2):
+        got = env.ctx.eval_level(pats, env.freq, mode=tfsm.MODE_EXACT, retain=True, level_mode=tfsm.LEVEL_GRAPH,
+                                 prune_batch=1)
+        assert env.ctx.last_stats().level_mode == tfsm.LEVEL_HOST
+        want = env.ctx.eval_level(pats, env.freq, mode=tfsm.MODE_EXACT, retain=True, level_mode=tfsm.LEVEL_HOST)
+        same_results(got, want, True, f"redone level with retained domains, round {rnd}")
+        kept = 0
+        for g, w, p in zip(got, want, pats):
+            assert bool(g.domain_handle) == bool(w.domain_handle)
+            if g.domain_handle:
+                for u in range(p.num_vertices):
+                    assert list(env.ctx.domain_read(g.domain_handle, u)) == list(env.ctx.domain_read(w.domain_handle, u))
+                kept += 1
+        for r in got + want:
+            if r.domain_handle:
+                env.ctx.domain_release(r.domain_handle)
+        assert kept > 0
+
+
 @pytest.mark.parametrize("level_mode", [tfsm.LEVEL_HOST, tfsm.LEVEL_DEVICE, tfsm.LEVEL_GRAPH],
                          ids=["host", "device", "graph"])
 @pytest.mark.parametrize("name", ["gnm2k_f40_m4", "powerlaw100k_f9000_m5"])
