@@ -617,7 +617,7 @@ double frac_value(const FracSum &f) {
 
 // Stats, per-pattern results and retained child domains of a finished call,
 // from the states, counters and flags read back (both drivers).
-// Buffers a device-driven call's last kernel filled with every pattern's
+// Buffers a device-driven call's sequence filled with every pattern's
 // domains (one per pattern, null when the host copies what it keeps):
 // finish_call takes those of the patterns it retains, and the rest go back
 // to the cache when the call ends, on every path.
@@ -642,7 +642,7 @@ struct Finished {
     uint32_t verify_launches;
     uint32_t level_mode;
     bool timed;                     // the stream events of every phase were recorded
-    KeptBuffers *kept;              // the device's copies of every domain, or null
+    KeptBuffers *kept;              // the sequence's copies of every domain, or null
 };
 
 int finish_call(tfsm_ctx *c, const EvalCall &call, const Finished &f) {
@@ -744,7 +744,7 @@ int finish_call(tfsm_ctx *c, const EvalCall &call, const Finished &f) {
             r.frequent = decided[p] != TFSM_DECIDED_FILTER && lo >= call.freq;
         }
         if (r.frequent && opt.retain_domains && f.kept) {
-            // the call's last kernel copied the domain here
+            // the sequence wrote the domain and its invalid bits here
             Retained R;
             R.nv = P.nv;
             R.buf = f.kept->buf[p].first;
@@ -956,8 +956,9 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
         }
     }
     // retained domains: when their bounds fit, every pattern gets a buffer
-    // for its domains at their source sizes, filled by the call's last
-    // kernel, so the call ends with no copies left to launch
+    // for its domains at their source sizes, filled in the sequence (the
+    // compaction writes the domains, its last kernel their invalid bits), so
+    // the call ends with no copies left to launch
     KeptBuffers kept(c);
     if (opt.retain_domains) {
         unsigned long long keep_words = 0;
@@ -1172,8 +1173,8 @@ int run_device(tfsm_ctx *c, const EvalCall &call, std::vector<PatStatic> &hs, co
     }
     const int rc = finish_call(c, call, Finished{hs, hst, hctr, hpool, decided, nullptr, total_tasks,
                                                  learned ? 2u : 1u, lm, !graph, sh.keep ? &kept : nullptr});
-    // the kept domains were written in the sequence, so nothing was queued
-    // after the sync unless the call retained nothing that way
+    // the stream is idle when the sequence wrote the kept domains or none
+    // were retained; otherwise finish_call queued their copies
     c->stream_idle = sh.keep || !opt.retain_domains;
     if (graph) {
         // the phase clock in ms: first block .. compaction .. tables .. search start .. end
