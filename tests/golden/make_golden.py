@@ -24,7 +24,10 @@ vertices between `~~~~~` / `#####` markers (parallel/comper.h:1027-1031,
 1095-1101) and only counts the size-2 patterns, so its fixtures hold the
 patterns of three or more vertices plus the reported total.  The
 Fraction-Score build (oracle/_ref/run_fraction, from fraction-score/) prints
-as the parallel build does.
+as the parallel build does.  Every reference run's console output is kept in
+tests/golden/_raw/<name>.stdout (not committed) before it is parsed: with
+several threads the parallel build can interleave its pattern lines, and the
+parse then stops with the line it could not read.
 """
 import fcntl
 import hashlib
@@ -166,6 +169,13 @@ def oracle_check(name, threads):
     return 0 if entry["oracle_check"]["equal"] else 1
 
 
+def keep_raw(name, text):
+    raw = os.path.join(HERE, "_raw")
+    os.makedirs(raw, exist_ok=True)
+    with open(os.path.join(raw, name + ".stdout"), "w") as f:
+        f.write(text)
+
+
 def main(names):
     manifest_path = os.path.join(HERE, "manifest.json")
     manifest = json.load(open(manifest_path)) if os.path.exists(manifest_path) else {}
@@ -177,6 +187,8 @@ def main(names):
             if build == "serial":
                 cmd = [os.path.join(REF, "run_serial"), "-file", lg, "-freq", str(freq), "-maxNodes", str(maxn)]
                 out = subprocess.run(cmd, cwd=tmp, check=True, capture_output=True, text=True).stdout
+                keep_raw(name, out)
+                keep_raw(name + ".grami_results", open(os.path.join(tmp, "grami_results.txt")).read())
                 keys = parse_serial(os.path.join(tmp, "grami_results.txt"))
                 if workload in ("readme", "gnm2k"):
                     # the reference's own output file, for the CLI output test
@@ -188,6 +200,7 @@ def main(names):
                 cmd = [os.path.join(REF, exe), "-file", lg, "-freq", str(freq), "-thread", str(threads),
                        "-maxNodes", str(maxn)]
                 out = subprocess.run(cmd, cwd=tmp, check=True, capture_output=True, text=True).stdout
+                keep_raw(name, out)
                 keys = parse_parallel(out)
                 reported = int([l for l in out.split("\n") if "[INFO] # Frequent Patterns" in l][0].split(":")[1])
             keys = sorted(set(keys))

@@ -39,3 +39,17 @@ def test_parallel_fixture_counts():
     assert len(keys) == m["keys"] == 330
     assert all(k.count(";") >= 2 for k in keys)
     assert m["reported_total"] == 334
+
+
+def test_parallel_fixture_matches_serial():
+    """The serial build mined the same configuration (all frequent patterns,
+    written by serial/grami.h:622-635): its patterns of three or more
+    vertices are exactly the parallel build's fixture, and its total is the
+    parallel build's reported total, so that fixture lost nothing on the
+    parallel build's timeout path."""
+    serial = golden_keys("powerlaw100k_f6000_m6_serial")
+    parallel = golden_keys("powerlaw100k_f6000_m6")
+    m = golden_manifest()
+    assert len(serial) == m["powerlaw100k_f6000_m6_serial"]["reported_total"] == 334
+    assert {k for k in serial if k.count(";") >= 2} == parallel
+    assert len(serial) == m["powerlaw100k_f6000_m6"]["reported_total"]
